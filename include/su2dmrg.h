@@ -1,0 +1,117 @@
+/*
+ * su2dmrg.h -- C ABI of the MI355X SU(2) DMRG hot path.
+ *
+ * Plain pointers and sizes only.  Every call returns an int status
+ * (SU2_OK = 0; see su2_status below) where the reference prints and calls
+ * exit(0) (e.g. tensor_su2.cpp:165-168, tensor_su2_dmrg_src.cpp:40-53);
+ * su2_last_error() returns the message of the last failure on a context.
+ *
+ * Conventions shared with the reference (lingwangqs/SU2DMRG):
+ *  - spins are doubled: twoj = 2j (tensor.cpp:1355);
+ *  - a leg is a list of sectors (twoj ascending, multiplet count) plus a
+ *    direction +-1 (su2bond.hpp:3-29);
+ *  - blocks are indexed mixed-radix over sector indices with leg 0 fastest,
+ *    and a block exists when sequential fusion of its labels reaches the
+ *    tensor's locspin (su2struct.cpp:243-332);
+ *  - block data are the reduced tensors, column-major with leg 0 fastest
+ *    (tensor.cpp:1101-1119), concatenated in block-index order -- the
+ *    serialisation of tensor_su2::get_telement(tele1,tele2,iflag)
+ *    (tensor_su2.cpp:532-549) without the CG part, which is implicit:
+ *    two-leg wavefunctions carry the unit singlet, two-leg operators the
+ *    identity, three-leg operators make_cgc(j0,j1,j2).
+ *
+ * Calls on one context are serialised on its stream and must not overlap
+ * from several host threads (the reference calls these kernels from inside
+ * OpenMP regions, dmrg_su2.cpp:582 -> tensor_su2_dmrg_src.cpp:977; here the
+ * caller hands the whole term list to one call instead).
+ */
+#ifndef SU2DMRG_H
+#define SU2DMRG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct su2_ctx su2_ctx;
+typedef struct su2_tensor su2_tensor;
+typedef struct su2_heff su2_heff;
+
+enum su2_status {
+  SU2_OK = 0,
+  SU2_ERR_ARGUMENT = 1,
+  SU2_ERR_STRUCTURE = 2,
+  SU2_ERR_DEVICE = 3,
+  SU2_ERR_CAPACITY = 4,
+  SU2_ERR_NUMERIC = 5
+};
+
+/* Library version string. */
+const char* su2_version(void);
+
+/* Context on one HIP device (replaces the globals psize/max_dcut/fac_* that
+ * tensor_su2.cpp:14-16 and tensor.cpp:25 read). */
+int su2_ctx_create(int device, su2_ctx** out);
+int su2_ctx_destroy(su2_ctx* ctx);
+const char* su2_last_error(const su2_ctx* ctx);
+int su2_ctx_synchronize(su2_ctx* ctx);
+
+/* Block-sparse tensor on the device: su2struct + reduced blocks
+ * (tensor_su2::set_tensor_su2(su2struct&, double*, double*, int*),
+ * tensor_su2.cpp:1431-1470).  twoj/dims hold nsec[0]+..+nsec[nleg-1]
+ * entries, leg after leg; present (nullable) holds one flag per block index
+ * (iflag of dmrg_su2_omp.cpp:864-941); blocks start zeroed. */
+int su2_tensor_create(su2_ctx* ctx, int nleg, const int* nsec, const int* twoj, const int* dims,
+                      const int* dirs, int locspin, const int* present, su2_tensor** out);
+int su2_tensor_destroy(su2_ctx* ctx, su2_tensor* t);
+/* Number of reduced doubles and of present blocks. */
+int su2_tensor_size(const su2_tensor* t, long long* ndouble, int* nblock);
+/* Block i (0..nblock-1, block-index order): block index, labels, dims, offset. */
+int su2_tensor_block(const su2_tensor* t, int i, int* index, int* twoj, int* dims,
+                     long long* offset);
+/* Host <-> device copies of all reduced doubles (n must equal the size). */
+int su2_tensor_upload(su2_ctx* ctx, su2_tensor* t, const double* host, long long n);
+int su2_tensor_download(su2_ctx* ctx, const su2_tensor* t, double* host, long long n);
+
+/* Lanczos vector algebra over the reduced blocks (tensor_su2::operator=,
+ * +=, -=, *=, /=, inner_prod, normalize_vector: tensor_su2.cpp:106-259,
+ * 324-356, 663-684).  Reductions are deterministic. */
+int su2_tensor_copy(su2_ctx* ctx, su2_tensor* dst, const su2_tensor* src);
+int su2_tensor_axpy(su2_ctx* ctx, su2_tensor* y, double a, const su2_tensor* x);
+int su2_tensor_scal(su2_ctx* ctx, su2_tensor* x, double a);
+int su2_tensor_dot(su2_ctx* ctx, const su2_tensor* x, const su2_tensor* y, double* out);
+int su2_tensor_normalize(su2_ctx* ctx, su2_tensor* x, double* norm);
+
+/* H_eff plan (dmrg_su2::hamiltonian_vector_multiplication_idmrg,
+ * dmrg_su2.cpp:543-664, and tensor_su2::hamiltonian_vector_multiplication,
+ * tensor_su2_dmrg_src.cpp:893-1041).
+ *   psi_layout : two-leg singlet wavefunction defining the block layout;
+ *   hl, hr     : block Hamiltonians hh[il], hh[ir] (nullable);
+ *   nterm      : number of S_L . S_R terms;
+ *   left_count[t] operators of term t are summed as
+ *     sum_i left_coef[i] * left_ops[i] (consecutive across terms), likewise
+ *     for the right side -- this is the coupling sum op = sum_k coup *
+ *     opr[ir][k] of dmrg_su2.cpp:585-599, formed once here.
+ * Operators are three-leg [ket, spin-1 leg (2j=2, one multiplet), bra]. */
+int su2_heff_create(su2_ctx* ctx, const su2_tensor* psi_layout, const su2_tensor* hl,
+                    const su2_tensor* hr, int nterm, const int* left_count,
+                    const su2_tensor* const* left_ops, const double* left_coef,
+                    const int* right_count, const su2_tensor* const* right_ops,
+                    const double* right_coef, su2_heff** out);
+/* out = H_eff psi (psi, out: the plan's layout; out may not alias psi). */
+int su2_heff_apply(su2_ctx* ctx, su2_heff* plan, const su2_tensor* psi, su2_tensor* out);
+/* Algorithmic FP64 flops of one apply in the reference's GEMM order
+ * (per term and block pair: step1 + step2, plus the hh products). */
+int su2_heff_flops(const su2_heff* plan, double* algorithmic, double* executed);
+int su2_heff_destroy(su2_ctx* ctx, su2_heff* plan);
+
+/* Time `reps` back-to-back applies on the context stream with HIP events
+ * (after one untimed apply); returns the mean milliseconds per apply and the
+ * mean milliseconds of the two GEMM stages measured separately. */
+int su2_heff_time(su2_ctx* ctx, su2_heff* plan, const su2_tensor* psi, su2_tensor* out, int reps,
+                  double* ms_apply, double* ms_stage1, double* ms_stage2);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SU2DMRG_H */

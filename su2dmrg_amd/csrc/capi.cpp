@@ -1,0 +1,348 @@
+// extern "C" entry points declared in include/su2dmrg.h.
+#include "../../include/su2dmrg.h"
+
+#include <cmath>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "blas1.hpp"
+#include "common.hpp"
+#include "context.hpp"
+#include "heff.hpp"
+#include "tensor.hpp"
+
+struct su2_ctx {
+  su2::Context c;
+  explicit su2_ctx(int dev) : c(dev) {}
+};
+struct su2_tensor {
+  su2::Tensor t;
+  template <class... A>
+  explicit su2_tensor(A&&... a) : t(std::forward<A>(a)...) {}
+};
+struct su2_heff {
+  su2::HeffPlan p;
+  template <class... A>
+  explicit su2_heff(A&&... a) : p(std::forward<A>(a)...) {}
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guard(su2_ctx* ctx, F&& f) {
+  try {
+    f();
+    return SU2_OK;
+  } catch (const su2::Error& e) {
+    if (ctx) ctx->c.last_error = e.what();
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    if (ctx) ctx->c.last_error = "host allocation failed";
+    g_err = "host allocation failed";
+    return SU2_ERR_CAPACITY;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->c.last_error = e.what();
+    g_err = e.what();
+    return SU2_ERR_ARGUMENT;
+  }
+}
+
+void need(const void* p, const char* what) {
+  SU2_CHECK(p != nullptr, su2::kErrArgument, "%s is NULL", what);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* su2_version(void) { return "su2dmrg-mi355x 0.1"; }
+
+int su2_ctx_create(int device, su2_ctx** out) {
+  return guard(nullptr, [&] {
+    need(out, "out");
+    *out = new su2_ctx(device);
+  });
+}
+
+int su2_ctx_destroy(su2_ctx* ctx) {
+  return guard(nullptr, [&] { delete ctx; });
+}
+
+const char* su2_last_error(const su2_ctx* ctx) {
+  return ctx ? ctx->c.last_error.c_str() : g_err.c_str();
+}
+
+int su2_ctx_synchronize(su2_ctx* ctx) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    ctx->c.make_current();
+    SU2_HIP(hipStreamSynchronize(ctx->c.stream()));
+  });
+}
+
+int su2_tensor_create(su2_ctx* ctx, int nleg, const int* nsec, const int* twoj, const int* dims,
+                      const int* dirs, int locspin, const int* present, su2_tensor** out) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(nsec, "nsec");
+    need(twoj, "twoj");
+    need(dims, "dims");
+    need(dirs, "dirs");
+    need(out, "out");
+    SU2_CHECK(nleg >= 1 && nleg <= su2::kMaxLegs, su2::kErrArgument, "nleg=%d", nleg);
+    std::vector<su2::Leg> legs;
+    int k = 0;
+    long long nten = 1;
+    for (int i = 0; i < nleg; ++i) {
+      SU2_CHECK(nsec[i] >= 1 && nsec[i] <= 64, su2::kErrArgument, "leg %d: %d sectors", i, nsec[i]);
+      std::vector<int> tj(twoj + k, twoj + k + nsec[i]), dm(dims + k, dims + k + nsec[i]);
+      for (size_t s = 1; s < tj.size(); ++s)
+        SU2_CHECK(tj[s] > tj[s - 1], su2::kErrArgument, "leg %d: 2j not strictly ascending", i);
+      legs.push_back(su2::make_leg(tj, dm, dirs[i]));
+      k += nsec[i];
+      nten *= nsec[i];
+    }
+    std::vector<char> pres;
+    if (present) {
+      pres.resize(nten);
+      for (long long i = 0; i < nten; ++i) pres[i] = present[i] != 0;
+    }
+    *out = new su2_tensor(&ctx->c, legs, locspin, present ? &pres : nullptr);
+  });
+}
+
+int su2_tensor_destroy(su2_ctx* ctx, su2_tensor* t) {
+  return guard(ctx, [&] {
+    if (ctx) ctx->c.make_current();
+    delete t;
+  });
+}
+
+int su2_tensor_size(const su2_tensor* t, long long* ndouble, int* nblock) {
+  return guard(nullptr, [&] {
+    need(t, "tensor");
+    if (ndouble) *ndouble = t->t.size();
+    if (nblock) *nblock = (int)t->t.blocks().size();
+  });
+}
+
+int su2_tensor_block(const su2_tensor* t, int i, int* index, int* twoj, int* dims,
+                     long long* offset) {
+  return guard(nullptr, [&] {
+    need(t, "tensor");
+    SU2_CHECK(i >= 0 && i < (int)t->t.blocks().size(), su2::kErrArgument, "block %d", i);
+    const su2::Block& b = t->t.blocks()[i];
+    if (index) *index = b.index;
+    for (int l = 0; l < t->t.nleg(); ++l) {
+      if (twoj) twoj[l] = b.twoj[l];
+      if (dims) dims[l] = b.dim[l];
+    }
+    if (offset) *offset = b.offset;
+  });
+}
+
+int su2_tensor_upload(su2_ctx* ctx, su2_tensor* t, const double* host, long long n) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(t, "tensor");
+    SU2_CHECK(n == t->t.size(), su2::kErrArgument, "upload: %lld doubles for a tensor of %lld", n,
+              t->t.size());
+    if (n == 0) return;
+    need(host, "host");
+    ctx->c.make_current();
+    SU2_HIP(hipMemcpyAsync(t->t.data(), host, n * sizeof(double), hipMemcpyHostToDevice,
+                           ctx->c.stream()));
+    SU2_HIP(hipStreamSynchronize(ctx->c.stream()));
+  });
+}
+
+int su2_tensor_download(su2_ctx* ctx, const su2_tensor* t, double* host, long long n) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(t, "tensor");
+    SU2_CHECK(n == t->t.size(), su2::kErrArgument, "download: %lld doubles for a tensor of %lld",
+              n, t->t.size());
+    if (n == 0) return;
+    need(host, "host");
+    ctx->c.make_current();
+    SU2_HIP(hipMemcpyAsync(host, t->t.data(), n * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->c.stream()));
+    SU2_HIP(hipStreamSynchronize(ctx->c.stream()));
+  });
+}
+
+static void same_layout(const su2::Tensor& a, const su2::Tensor& b, const char* op) {
+  SU2_CHECK(a.same_layout(b), su2::kErrStructure, "%s: tensors have different SU(2) structure",
+            op);
+}
+
+int su2_tensor_copy(su2_ctx* ctx, su2_tensor* dst, const su2_tensor* src) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(dst, "dst");
+    need(src, "src");
+    same_layout(dst->t, src->t, "copy");
+    ctx->c.make_current();
+    su2::dev_copy(ctx->c, dst->t.data(), src->t.data(), src->t.size());
+  });
+}
+
+int su2_tensor_axpy(su2_ctx* ctx, su2_tensor* y, double a, const su2_tensor* x) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(y, "y");
+    need(x, "x");
+    same_layout(y->t, x->t, "axpy");
+    ctx->c.make_current();
+    su2::dev_axpy(ctx->c, y->t.data(), x->t.data(), x->t.size(), a);
+  });
+}
+
+int su2_tensor_scal(su2_ctx* ctx, su2_tensor* x, double a) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(x, "x");
+    ctx->c.make_current();
+    su2::dev_scal(ctx->c, x->t.data(), x->t.size(), a);
+  });
+}
+
+int su2_tensor_dot(su2_ctx* ctx, const su2_tensor* x, const su2_tensor* y, double* out) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(x, "x");
+    need(y, "y");
+    need(out, "out");
+    same_layout(x->t, y->t, "dot");
+    ctx->c.make_current();
+    *out = su2::dev_dot(ctx->c, x->t.data(), y->t.data(), x->t.size());
+  });
+}
+
+int su2_tensor_normalize(su2_ctx* ctx, su2_tensor* x, double* norm) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(x, "x");
+    ctx->c.make_current();
+    double n = std::sqrt(su2::dev_dot(ctx->c, x->t.data(), x->t.data(), x->t.size()));
+    SU2_CHECK(n > 0, su2::kErrNumeric, "normalize: zero vector");
+    su2::dev_scal(ctx->c, x->t.data(), x->t.size(), 1.0 / n);
+    if (norm) *norm = n;
+  });
+}
+
+int su2_heff_create(su2_ctx* ctx, const su2_tensor* psi_layout, const su2_tensor* hl,
+                    const su2_tensor* hr, int nterm, const int* left_count,
+                    const su2_tensor* const* left_ops, const double* left_coef,
+                    const int* right_count, const su2_tensor* const* right_ops,
+                    const double* right_coef, su2_heff** out) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(psi_layout, "psi_layout");
+    need(out, "out");
+    SU2_CHECK(nterm >= 0, su2::kErrArgument, "nterm=%d", nterm);
+    std::vector<su2::HeffTerm> terms(nterm);
+    if (nterm > 0) {
+      need(left_count, "left_count");
+      need(right_count, "right_count");
+      need(left_ops, "left_ops");
+      need(right_ops, "right_ops");
+      need(left_coef, "left_coef");
+      need(right_coef, "right_coef");
+    }
+    int li = 0, ri = 0;
+    for (int t = 0; t < nterm; ++t) {
+      SU2_CHECK(left_count[t] >= 1 && right_count[t] >= 1, su2::kErrArgument,
+                "term %d: empty side", t);
+      for (int i = 0; i < left_count[t]; ++i, ++li) {
+        need(left_ops[li], "left operator");
+        terms[t].left.push_back({left_coef[li], &left_ops[li]->t});
+      }
+      for (int i = 0; i < right_count[t]; ++i, ++ri) {
+        need(right_ops[ri], "right operator");
+        terms[t].right.push_back({right_coef[ri], &right_ops[ri]->t});
+      }
+    }
+    *out = new su2_heff(ctx->c, psi_layout->t, hl ? &hl->t : nullptr, hr ? &hr->t : nullptr,
+                        terms);
+  });
+}
+
+int su2_heff_apply(su2_ctx* ctx, su2_heff* plan, const su2_tensor* psi, su2_tensor* out) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(plan, "plan");
+    need(psi, "psi");
+    need(out, "out");
+    SU2_CHECK(psi != out, su2::kErrArgument, "heff_apply: out aliases psi");
+    plan->p.apply(psi->t, out->t);
+  });
+}
+
+int su2_heff_flops(const su2_heff* plan, double* algorithmic, double* executed) {
+  return guard(nullptr, [&] {
+    need(plan, "plan");
+    if (algorithmic) *algorithmic = plan->p.algorithmic_flops();
+    if (executed) *executed = plan->p.executed_flops();
+  });
+}
+
+int su2_heff_destroy(su2_ctx* ctx, su2_heff* plan) {
+  return guard(ctx, [&] {
+    if (ctx) ctx->c.make_current();
+    delete plan;
+  });
+}
+
+int su2_heff_time(su2_ctx* ctx, su2_heff* plan, const su2_tensor* psi, su2_tensor* out, int reps,
+                  double* ms_apply, double* ms_stage1, double* ms_stage2) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(plan, "plan");
+    need(psi, "psi");
+    need(out, "out");
+    SU2_CHECK(reps >= 1, su2::kErrArgument, "reps=%d", reps);
+    ctx->c.make_current();
+    hipStream_t s = ctx->c.stream();
+    plan->p.apply(psi->t, out->t);
+    hipEvent_t e0, e1, e2, e3;
+    SU2_HIP(hipEventCreate(&e0));
+    SU2_HIP(hipEventCreate(&e1));
+    SU2_HIP(hipEventCreate(&e2));
+    SU2_HIP(hipEventCreate(&e3));
+    SU2_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) plan->p.apply(psi->t, out->t);
+    SU2_HIP(hipEventRecord(e1, s));
+    SU2_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    SU2_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms_apply) *ms_apply = ms / reps;
+    // the two stages alone
+    float m1 = 0, m2 = 0;
+    for (int r = 0; r < reps; ++r) {
+      float a = 0, b = 0;
+      SU2_HIP(hipEventRecord(e0, s));
+      plan->p.stage1().launch(s);
+      SU2_HIP(hipEventRecord(e2, s));
+      plan->p.stage2().launch(s);
+      SU2_HIP(hipEventRecord(e3, s));
+      SU2_HIP(hipEventSynchronize(e3));
+      SU2_HIP(hipEventElapsedTime(&a, e0, e2));
+      SU2_HIP(hipEventElapsedTime(&b, e2, e3));
+      m1 += a;
+      m2 += b;
+    }
+    if (ms_stage1) *ms_stage1 = m1 / reps;
+    if (ms_stage2) *ms_stage2 = m2 / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    (void)hipEventDestroy(e3);
+  });
+}
+
+}  // extern "C"

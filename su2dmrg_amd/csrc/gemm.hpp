@@ -1,0 +1,88 @@
+// Grouped FP64 GEMM on CDNA4 matrix cores (v_mfma_f64_16x16x4_f64).
+//
+// One launch runs a list of independent column-major problems
+//   C_p[M_p x N_p] = sum_s alpha_s * op(A_s)[M_p x K_s] * op(B_s)[K_s x N_p]
+// where each problem owns a contiguous list of K-slices (the "sum over
+// terms and sectors" of the H_eff matvec becomes one long K loop).  A problem
+// may be split along its concatenated K range; split partials go to a
+// workspace and a second kernel adds them in a fixed order, so results are
+// deterministic (no atomics).  The recoupling scalar alpha_s is applied to A
+// as it is staged through LDS.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+namespace su2 {
+
+struct GemmSlice {
+  const double* A;
+  const double* B;
+  long long lda, ldb;
+  int K;
+  int transA, transB;
+  int kt_begin;      // first k-tile index of this slice inside its problem
+  double alpha;
+};
+
+struct GemmProblem {
+  double* C;
+  long long ldc;
+  int M, N;
+  int slice_begin, slice_end;
+  int tiles_m, tiles_n;
+  int ksplit;
+  int kt_total;      // k-tiles over all slices (each slice padded to the tile)
+  long long ws_offset;  // doubles into the split-K workspace (ksplit > 1)
+  long long unit_begin;
+};
+
+struct ReduceProblem {
+  double* C;
+  long long ldc;
+  int M, N, ksplit;
+  long long ws_offset;
+  long long elem_begin;
+};
+
+// Host-side plan: build problems, then upload once; launch many times.
+class GroupedGemm {
+ public:
+  GroupedGemm() = default;
+  ~GroupedGemm();
+  GroupedGemm(const GroupedGemm&) = delete;
+  GroupedGemm& operator=(const GroupedGemm&) = delete;
+
+  // Adds a problem; returns its index.  Slices are added right after.
+  int add_problem(double* C, long long ldc, int M, int N);
+  void add_slice(int problem, const double* A, long long lda, int transA,
+                 const double* B, long long ldb, int transB, int K, double alpha);
+  // Chooses K splits for a target number of work units and uploads tables.
+  void finalize(int target_units);
+  void launch(hipStream_t stream) const;
+
+  double flops() const { return flops_; }
+  long long units() const { return total_units_; }
+  size_t nproblems() const { return probs_.size(); }
+
+ private:
+  std::vector<GemmProblem> probs_;
+  std::vector<GemmSlice> slices_;
+  std::vector<ReduceProblem> reds_;
+  GemmProblem* d_probs_ = nullptr;
+  GemmSlice* d_slices_ = nullptr;
+  ReduceProblem* d_reds_ = nullptr;
+  double* d_ws_ = nullptr;
+  long long total_units_ = 0;
+  long long total_red_elems_ = 0;
+  double flops_ = 0;
+  bool finalized_ = false;
+};
+
+// Tile shape of the FP64 MFMA kernel.
+constexpr int kGemmBM = 128;
+constexpr int kGemmBN = 128;
+constexpr int kGemmBK = 16;
+
+}  // namespace su2

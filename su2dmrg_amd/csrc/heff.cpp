@@ -1,0 +1,257 @@
+#include "heff.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+
+#include "blas1.hpp"
+#include "cg.hpp"
+#include "common.hpp"
+#include "context.hpp"
+
+namespace su2 {
+
+namespace {
+
+constexpr int kOpTwoj = 2;            // spin-1 operator leg
+constexpr double kFacSkip = 1e-12;    // tensor_su2_dmrg_src.cpp:996
+constexpr int kTargetUnits = 2048;    // ~8 work units per CU on 256 CUs
+
+void check_op(const Tensor& op, const Leg& in, const Leg& out, const char* what) {
+  SU2_CHECK(op.nleg() == 3, kErrStructure, "%s operator must have 3 legs", what);
+  SU2_CHECK(op.leg(0).same_sectors(in), kErrStructure,
+            "%s operator leg 0 does not match the wavefunction leg", what);
+  SU2_CHECK(op.leg(2).same_sectors(out), kErrStructure,
+            "%s operator leg 2 does not match the wavefunction leg", what);
+  SU2_CHECK(op.leg(1).nsec() == 1 && op.leg(1).twoj[0] == kOpTwoj && op.leg(1).dim[0] == 1,
+            kErrStructure, "%s operator leg 1 must be a single spin-1 multiplet", what);
+}
+
+void check_scalar(const Tensor& h, const Leg& l, const char* what) {
+  SU2_CHECK(h.nleg() == 2, kErrStructure, "%s must have 2 legs", what);
+  SU2_CHECK(h.leg(0).same_sectors(l) && h.leg(1).same_sectors(l), kErrStructure,
+            "%s legs do not match the wavefunction leg", what);
+}
+
+}  // namespace
+
+HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor* hr,
+                   const std::vector<HeffTerm>& terms)
+    : ctx_(c) {
+  c.make_current();
+  SU2_CHECK(psi.nleg() == 2 && psi.locspin() == 0, kErrStructure,
+            "H_eff wavefunction must be a two-leg singlet tensor");
+  const Leg& la = psi.leg(0);
+  const Leg& lb = psi.leg(1);
+  if (hl) check_scalar(*hl, la, "left block Hamiltonian");
+  if (hr) check_scalar(*hr, lb, "right block Hamiltonian");
+  const int T = (int)terms.size();
+  for (auto& t : terms) {
+    SU2_CHECK(!t.left.empty() && !t.right.empty(), kErrArgument, "H_eff term with an empty side");
+    for (auto& p : t.left) check_op(*p.second, la, la, "left");
+    for (auto& p : t.right) check_op(*p.second, lb, lb, "right");
+  }
+  psi_size_ = psi.size();
+
+  // psi block lookup by 2j
+  std::map<int, const Block*> pb;
+  for (auto& b : psi.blocks()) pb[b.twoj[0]] = &b;
+
+  // transitions (j -> j') present on both sides of at least one term
+  struct Tr {
+    int ja, jap;
+    double fac;
+    int Da, Dap, Db, Dbp;
+    long long lb_off, rb_off, x_off;
+  };
+  std::vector<Tr> trs;
+  {
+    std::map<std::pair<int, int>, int> seen;
+    for (auto& t : terms)
+      for (auto& p : t.left)
+        for (auto& blk : p.second->blocks()) {
+          const int ja = blk.twoj[0], jap = blk.twoj[2];
+          if (!pb.count(ja) || !pb.count(jap)) continue;
+          if (seen.count({ja, jap})) continue;
+          const double f = heff_recoupling(ja, kOpTwoj, jap);
+          if (std::fabs(f) < kFacSkip) continue;
+          bool right_has = false;
+          for (auto& tt : terms)
+            for (auto& q : tt.right)
+              if (q.second->find3(ja, kOpTwoj, jap) >= 0) right_has = true;
+          if (!right_has) continue;
+          seen[{ja, jap}] = 1;
+          Tr tr{};
+          tr.ja = ja;
+          tr.jap = jap;
+          tr.fac = f;
+          tr.Da = la.dim_of(ja);
+          tr.Dap = la.dim_of(jap);
+          tr.Db = lb.dim_of(ja);
+          tr.Dbp = lb.dim_of(jap);
+          trs.push_back(tr);
+        }
+    std::sort(trs.begin(), trs.end(), [](const Tr& x, const Tr& y) {
+      return x.jap != y.jap ? x.jap < y.jap : x.ja < y.ja;
+    });
+  }
+
+  // algorithmic flops in the reference's order (per term, per block present
+  // on both sides), plus the block-Hamiltonian products
+  for (auto& t : terms) {
+    for (auto& tr : trs) {
+      bool l = false, r = false;
+      for (auto& p : t.left) l = l || p.second->find3(tr.ja, kOpTwoj, tr.jap) >= 0;
+      for (auto& p : t.right) r = r || p.second->find3(tr.ja, kOpTwoj, tr.jap) >= 0;
+      if (l && r)
+        alg_flops_ += 2.0 * tr.Db * tr.Da * tr.Dap + 2.0 * tr.Dap * tr.Db * tr.Dbp;
+    }
+  }
+  for (auto& b : psi.blocks()) {
+    const double da = b.dim[0], db = b.dim[1];
+    if (hl && hl->find2(b.twoj[0], b.twoj[0]) >= 0) alg_flops_ += 2.0 * da * da * db;
+    if (hr && hr->find2(b.twoj[1], b.twoj[1]) >= 0) alg_flops_ += 2.0 * da * db * db;
+  }
+
+  // workspace: bundles and stage-1 outputs
+  long long work = 0;
+  for (auto& tr : trs) {
+    tr.lb_off = work;
+    work += (long long)tr.Da * tr.Dap * T;
+    tr.rb_off = work;
+    work += (long long)T * tr.Db * tr.Dbp;
+    tr.x_off = work;
+    work += (long long)tr.Dap * T * tr.Db;
+  }
+  SU2_HIP(hipMalloc(&d_in_, std::max<long long>(psi_size_, 1) * sizeof(double)));
+  SU2_HIP(hipMalloc(&d_out_, std::max<long long>(psi_size_, 1) * sizeof(double)));
+  if (work > 0) SU2_HIP(hipMalloc(&d_work_, work * sizeof(double)));
+  dev_fill(c, d_work_, work, 0.0);
+
+  // bundle assembly: one batched strided-axpy launch per summand position, so
+  // coupling sums accumulate in the order given (deterministic)
+  size_t maxside = 0;
+  for (auto& t : terms) maxside = std::max({maxside, t.left.size(), t.right.size()});
+  for (size_t pos = 0; pos < maxside; ++pos) {
+    std::vector<StridedAxpy> desc;
+    long long total = 0;
+    for (int ti = 0; ti < T; ++ti) {
+      const HeffTerm& t = terms[ti];
+      for (auto& tr : trs) {
+        if (pos < t.left.size()) {
+          const Tensor& op = *t.left[pos].second;
+          int id = op.find3(tr.ja, kOpTwoj, tr.jap);
+          if (id >= 0) {
+            StridedAxpy d{};
+            d.dst = d_work_ + tr.lb_off + (long long)ti * tr.Da * tr.Dap;
+            d.src = op.data() + op.blocks()[id].offset;
+            d.rs = 1;
+            d.cs = tr.Da;
+            d.ld = tr.Da;
+            d.rows = tr.Da;
+            d.cols = tr.Dap;
+            d.alpha = t.left[pos].first;
+            d.elem_begin = total;
+            total += (long long)tr.Da * tr.Dap;
+            desc.push_back(d);
+          }
+        }
+        if (pos < t.right.size()) {
+          const Tensor& op = *t.right[pos].second;
+          int id = op.find3(tr.ja, kOpTwoj, tr.jap);
+          if (id >= 0) {
+            StridedAxpy d{};
+            d.dst = d_work_ + tr.rb_off + ti;
+            d.src = op.data() + op.blocks()[id].offset;
+            d.rs = T;
+            d.cs = (long long)T * tr.Db;
+            d.ld = tr.Db;
+            d.rows = tr.Db;
+            d.cols = tr.Dbp;
+            d.alpha = t.right[pos].first;
+            d.elem_begin = total;
+            total += (long long)tr.Db * tr.Dbp;
+            desc.push_back(d);
+          }
+        }
+      }
+    }
+    if (desc.empty()) continue;
+    StridedAxpy* d_desc = nullptr;
+    SU2_HIP(hipMalloc(&d_desc, desc.size() * sizeof(StridedAxpy)));
+    SU2_HIP(hipMemcpyAsync(d_desc, desc.data(), desc.size() * sizeof(StridedAxpy),
+                           hipMemcpyHostToDevice, c.stream()));
+    dev_strided_axpy(c, d_desc, (int)desc.size(), total);
+    SU2_HIP(hipStreamSynchronize(c.stream()));
+    SU2_HIP(hipFree(d_desc));
+  }
+
+  // stage 1: X(j->j') = f * Lb^T psi(j)   [T*D_a(j') x D_b(j)]
+  for (auto& tr : trs) {
+    if (T == 0) break;
+    const Block* bj = pb[tr.ja];
+    int p = stage1_.add_problem(d_work_ + tr.x_off, (long long)T * tr.Dap, T * tr.Dap, tr.Db);
+    stage1_.add_slice(p, d_work_ + tr.lb_off, tr.Da, 1, d_in_ + bj->offset, tr.Da, 0, tr.Da,
+                      tr.fac);
+  }
+  // stage 2: psi'(j') = sum_j X(j->j') Rb(j->j') + H_L^T psi + psi H_R
+  for (auto& b : psi.blocks()) {
+    const int jap = b.twoj[0];
+    const int Dap = b.dim[0], Dbp = b.dim[1];
+    bool any = false;
+    int p = -1;
+    auto ensure = [&]() {
+      if (p < 0) p = stage2_.add_problem(d_out_ + b.offset, Dap, Dap, Dbp);
+      any = true;
+    };
+    for (auto& tr : trs) {
+      if (tr.jap != jap || T == 0) continue;
+      ensure();
+      stage2_.add_slice(p, d_work_ + tr.x_off, Dap, 0, d_work_ + tr.rb_off, (long long)T * tr.Db,
+                        0, T * tr.Db, 1.0);
+    }
+    if (hl) {
+      int id = hl->find2(jap, jap);
+      if (id >= 0) {
+        ensure();
+        stage2_.add_slice(p, hl->data() + hl->blocks()[id].offset, Dap, 1, d_in_ + b.offset, Dap,
+                          0, Dap, 1.0);
+      }
+    }
+    if (hr) {
+      int id = hr->find2(b.twoj[1], b.twoj[1]);
+      if (id >= 0) {
+        ensure();
+        stage2_.add_slice(p, d_in_ + b.offset, Dap, 0, hr->data() + hr->blocks()[id].offset, Dbp,
+                          0, Dbp, 1.0);
+      }
+    }
+    if (!any) zero_blocks_.push_back({b.offset, b.size});
+  }
+  stage1_.finalize(kTargetUnits);
+  stage2_.finalize(kTargetUnits);
+  SU2_HIP(hipStreamSynchronize(c.stream()));
+}
+
+HeffPlan::~HeffPlan() {
+  if (d_in_) (void)hipFree(d_in_);
+  if (d_out_) (void)hipFree(d_out_);
+  if (d_work_) (void)hipFree(d_work_);
+}
+
+void HeffPlan::apply_raw(const double* psi, double* out) {
+  ctx_.make_current();
+  dev_copy(ctx_, d_in_, psi, psi_size_);
+  for (auto& z : zero_blocks_) dev_fill(ctx_, d_out_ + z.first, z.second, 0.0);
+  stage1_.launch(ctx_.stream());
+  stage2_.launch(ctx_.stream());
+  dev_copy(ctx_, out, d_out_, psi_size_);
+}
+
+void HeffPlan::apply(const Tensor& psi, Tensor& out) {
+  SU2_CHECK(psi.size() == psi_size_ && out.size() == psi_size_, kErrArgument,
+            "H_eff apply: vector layout differs from the plan");
+  apply_raw(psi.data(), out.data());
+}
+
+}  // namespace su2

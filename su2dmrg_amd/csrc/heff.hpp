@@ -1,0 +1,64 @@
+// H_eff matvec plan: the block-sparse two-site effective Hamiltonian applied
+// to a two-leg SU(2) wavefunction, as dmrg_su2::hamiltonian_vector_
+// multiplication_idmrg computes it (dmrg_su2.cpp:543-664):
+//
+//   psi'(j') = H_L(j')^T psi(j') + psi(j') H_R(j')
+//            + sum_t sum_j f(j,j') L_t(j->j')^T psi(j) R_t(j->j')
+//
+// with f = the recoupling scalar of tensor_su2::hamiltonian_vector_
+// multiplication (tensor_su2_dmrg_src.cpp:893-1041).  Each term side may be a
+// coupling-weighted sum of operators (the reference re-sums the right
+// operators on every call, dmrg_su2.cpp:585-599; here the sum is formed once
+// when the plan is built).
+//
+// Execution: the T terms are packed into per-transition bundles
+//   Lb(j->j') = [D_a(j) x D_a(j')*T]   (columns a' fastest, then t)
+//   Rb(j->j') = [T*D_b(j) x D_b(j')]   (rows t fastest, then b)
+// so that stage 1 is one GEMM per transition, X = f * Lb^T psi(j), and stage 2
+// is one K-concatenated GEMM per output sector that also absorbs the two
+// block-Hamiltonian products.  Both stages are single grouped launches.
+#pragma once
+
+#include <utility>
+#include <vector>
+
+#include "gemm.hpp"
+#include "tensor.hpp"
+
+namespace su2 {
+
+struct HeffTerm {
+  std::vector<std::pair<double, const Tensor*>> left, right;
+};
+
+class HeffPlan {
+ public:
+  HeffPlan(Context& c, const Tensor& psi_layout, const Tensor* hl, const Tensor* hr,
+           const std::vector<HeffTerm>& terms);
+  ~HeffPlan();
+  HeffPlan(const HeffPlan&) = delete;
+  HeffPlan& operator=(const HeffPlan&) = delete;
+
+  // out = H psi; psi and out must have the layout the plan was built for.
+  void apply(const Tensor& psi, Tensor& out);
+  // Same, on raw device buffers of that layout (used by the Lanczos driver).
+  void apply_raw(const double* psi, double* out);
+
+  double algorithmic_flops() const { return alg_flops_; }
+  double executed_flops() const { return stage1_.flops() + stage2_.flops(); }
+  long long psi_size() const { return psi_size_; }
+  const GroupedGemm& stage1() const { return stage1_; }
+  const GroupedGemm& stage2() const { return stage2_; }
+
+ private:
+  Context& ctx_;
+  long long psi_size_ = 0;
+  double* d_in_ = nullptr;    // plan-owned copy of the input vector
+  double* d_out_ = nullptr;   // plan-owned output vector
+  double* d_work_ = nullptr;  // bundles + stage-1 intermediates
+  std::vector<std::pair<long long, long long>> zero_blocks_;  // (offset, size) without terms
+  GroupedGemm stage1_, stage2_;
+  double alg_flops_ = 0;
+};
+
+}  // namespace su2

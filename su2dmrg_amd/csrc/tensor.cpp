@@ -1,0 +1,120 @@
+#include "tensor.hpp"
+
+#include "common.hpp"
+#include "context.hpp"
+
+namespace su2 {
+
+Context::Context(int device) : device_(device) {
+  int n = 0;
+  SU2_HIP(hipGetDeviceCount(&n));
+  SU2_CHECK(device >= 0 && device < n, kErrArgument, "device %d out of range (%d devices)", device,
+            n);
+  SU2_HIP(hipSetDevice(device));
+  // default priority stream (never a prioritised one)
+  SU2_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  own_stream_ = true;
+}
+
+Context::~Context() {
+  (void)hipSetDevice(device_);
+  if (d_red_) (void)hipFree(d_red_);
+  if (h_red_) (void)hipHostFree(h_red_);
+  if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Context::make_current() const { SU2_HIP(hipSetDevice(device_)); }
+
+double* Context::reduce_scratch(size_t n) {
+  if (n > d_red_n_) {
+    if (d_red_) SU2_HIP(hipFree(d_red_));
+    size_t m = std::max(n, d_red_n_ * 2);
+    SU2_HIP(hipMalloc(&d_red_, m * sizeof(double)));
+    d_red_n_ = m;
+  }
+  return d_red_;
+}
+
+double* Context::host_scratch(size_t n) {
+  if (n > h_red_n_) {
+    if (h_red_) SU2_HIP(hipHostFree(h_red_));
+    size_t m = std::max(n, h_red_n_ * 2);
+    SU2_HIP(hipHostMalloc(&h_red_, m * sizeof(double), hipHostMallocDefault));
+    h_red_n_ = m;
+  }
+  return h_red_;
+}
+
+static long long key_of(const int* twoj, int n) {
+  long long k = 0;
+  for (int i = 0; i < n; ++i) k = k * 64 + twoj[i];
+  return k;
+}
+
+Tensor::Tensor(Context* ctx, std::vector<Leg> legs, int locspin, const std::vector<char>* present)
+    : ctx_(ctx), legs_(std::move(legs)), locspin_(locspin) {
+  const int n = (int)legs_.size();
+  SU2_CHECK(n >= 1 && n <= kMaxLegs, kErrArgument, "tensor: %d legs", n);
+  SU2_CHECK(locspin >= 0, kErrArgument, "tensor: negative locspin");
+  for (auto& l : legs_) {
+    SU2_CHECK(l.dir == 1 || l.dir == -1, kErrArgument, "tensor: leg direction must be +-1");
+    nten_ *= std::max(1, l.nsec());
+    for (int t : l.twoj) SU2_CHECK(t < 64, kErrArgument, "tensor: 2j=%d too large", t);
+  }
+  for (auto& l : legs_) SU2_CHECK(l.nsec() > 0, kErrArgument, "tensor: empty leg");
+  if (present) SU2_CHECK((int)present->size() == nten_, kErrArgument,
+                         "tensor: %zu present flags for %d blocks", present->size(), nten_);
+  long long off = 0;
+  for (int idx = 0; idx < nten_; ++idx) {
+    Block b;
+    int r = idx;
+    for (int i = 0; i < n; ++i) {
+      int s = r % legs_[i].nsec();
+      r /= legs_[i].nsec();
+      b.sec[i] = s;
+      b.twoj[i] = legs_[i].twoj[s];
+      b.dim[i] = legs_[i].dim[s];
+    }
+    bool ok = labels_allowed(b.twoj, n, locspin);
+    if (present) {
+      if ((*present)[idx] && !ok)
+        fail(kErrStructure, "tensor: block %d flagged present but its labels are not allowed", idx);
+      ok = ok && (*present)[idx];
+    }
+    if (!ok) continue;
+    b.index = idx;
+    b.size = 1;
+    for (int i = 0; i < n; ++i) b.size *= b.dim[i];
+    b.offset = off;
+    off += b.size;
+    by_key_[key_of(b.twoj, n)] = (int)blocks_.size();
+    blocks_.push_back(b);
+  }
+  size_ = off;
+  ctx_->make_current();
+  if (size_ > 0) {
+    SU2_HIP(hipMalloc(&d_, size_ * sizeof(double)));
+    SU2_HIP(hipMemsetAsync(d_, 0, size_ * sizeof(double), ctx_->stream()));
+  }
+}
+
+Tensor::~Tensor() {
+  if (d_) (void)hipFree(d_);
+}
+
+int Tensor::find(const int* twoj) const {
+  auto it = by_key_.find(key_of(twoj, (int)legs_.size()));
+  return it == by_key_.end() ? -1 : it->second;
+}
+
+bool Tensor::same_layout(const Tensor& o) const {
+  if (legs_.size() != o.legs_.size() || locspin_ != o.locspin_) return false;
+  for (size_t i = 0; i < legs_.size(); ++i)
+    if (!(legs_[i] == o.legs_[i])) return false;
+  if (blocks_.size() != o.blocks_.size()) return false;
+  for (size_t i = 0; i < blocks_.size(); ++i)
+    if (blocks_[i].index != o.blocks_[i].index) return false;
+  return true;
+}
+
+}  // namespace su2

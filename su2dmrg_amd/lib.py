@@ -1,0 +1,315 @@
+"""ctypes binding of ``libsu2dmrg.so`` (the C ABI in ``include/su2dmrg.h``).
+
+The library is the product: this module only loads it, declares the entry
+points and converts the plain Python block layout used by the tests and the
+benchmark (``2j -> ndarray`` wavefunctions, ``(2j_in, 2j_out) -> ndarray``
+operators) to and from the C ABI's flat block-index order.  There is no
+fallback: if the shared library is missing, importing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsu2dmrg.so")
+HEADER = os.path.join(HERE, "..", "include", "su2dmrg.h")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built; run `python -m su2dmrg_amd.build`")
+
+lib = C.CDLL(LIB_PATH)
+
+_p = C.c_void_p
+_i = C.c_int
+_ll = C.c_longlong
+_d = C.c_double
+_ip = C.POINTER(C.c_int)
+_dp = C.POINTER(C.c_double)
+_llp = C.POINTER(C.c_longlong)
+
+_SIG = {
+    "su2_version": (C.c_char_p, []),
+    "su2_ctx_create": (_i, [_i, C.POINTER(_p)]),
+    "su2_ctx_destroy": (_i, [_p]),
+    "su2_last_error": (C.c_char_p, [_p]),
+    "su2_ctx_synchronize": (_i, [_p]),
+    "su2_tensor_create": (_i, [_p, _i, _ip, _ip, _ip, _ip, _i, _ip, C.POINTER(_p)]),
+    "su2_tensor_destroy": (_i, [_p, _p]),
+    "su2_tensor_size": (_i, [_p, _llp, _ip]),
+    "su2_tensor_block": (_i, [_p, _i, _ip, _ip, _ip, _llp]),
+    "su2_tensor_upload": (_i, [_p, _p, _dp, _ll]),
+    "su2_tensor_download": (_i, [_p, _p, _dp, _ll]),
+    "su2_tensor_copy": (_i, [_p, _p, _p]),
+    "su2_tensor_axpy": (_i, [_p, _p, _d, _p]),
+    "su2_tensor_scal": (_i, [_p, _p, _d]),
+    "su2_tensor_dot": (_i, [_p, _p, _p, _dp]),
+    "su2_tensor_normalize": (_i, [_p, _p, _dp]),
+    "su2_heff_create": (_i, [_p, _p, _p, _p, _i, _ip, C.POINTER(_p), _dp, _ip, C.POINTER(_p),
+                             _dp, C.POINTER(_p)]),
+    "su2_heff_apply": (_i, [_p, _p, _p, _p]),
+    "su2_heff_flops": (_i, [_p, _dp, _dp]),
+    "su2_heff_destroy": (_i, [_p, _p]),
+    "su2_heff_time": (_i, [_p, _p, _p, _p, _i, _dp, _dp, _dp]),
+}
+
+for _name, (_res, _args) in _SIG.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+def declared_symbols():
+    """Entry points declared in include/su2dmrg.h."""
+    with open(HEADER) as fh:
+        text = fh.read()
+    return sorted(set(re.findall(r"\b(su2_[a-z0-9_]+)\s*\(", text)))
+
+
+class Su2Error(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"su2 status {code}: {msg}")
+        self.code = code
+
+
+def _ia(x):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.int32))
+    return a, a.ctypes.data_as(_ip)
+
+
+class Context:
+    """One HIP device + stream (su2_ctx)."""
+
+    def __init__(self, device=0):
+        h = _p()
+        rc = lib.su2_ctx_create(device, C.byref(h))
+        if rc:
+            raise Su2Error(rc, lib.su2_last_error(None).decode())
+        self.h = h
+
+    def check(self, rc):
+        if rc:
+            raise Su2Error(rc, lib.su2_last_error(self.h).decode())
+
+    def sync(self):
+        self.check(lib.su2_ctx_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            lib.su2_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Tensor:
+    """Device block-sparse tensor (su2_tensor)."""
+
+    def __init__(self, ctx, legs, locspin=0, present=None):
+        """``legs`` = list of (twoj list, dims list, dir)."""
+        self.ctx = ctx
+        self.legs = [(list(t), list(d), int(s)) for t, d, s in legs]
+        nsec = [len(t) for t, _, _ in self.legs]
+        twoj = [x for t, _, _ in self.legs for x in t]
+        dims = [x for _, d, _ in self.legs for x in d]
+        dirs = [s for _, _, s in self.legs]
+        a1, p1 = _ia(nsec)
+        a2, p2 = _ia(twoj)
+        a3, p3 = _ia(dims)
+        a4, p4 = _ia(dirs)
+        if present is not None:
+            a5, p5 = _ia(present)
+        else:
+            p5 = None
+        h = _p()
+        ctx.check(lib.su2_tensor_create(ctx.h, len(self.legs), p1, p2, p3, p4, locspin, p5,
+                                        C.byref(h)))
+        self.h = h
+        n = _ll()
+        nb = _i()
+        ctx.check(lib.su2_tensor_size(h, C.byref(n), C.byref(nb)))
+        self.size = n.value
+        self.blocks = []
+        nl = len(self.legs)
+        for i in range(nb.value):
+            idx = _i()
+            tj = (C.c_int * nl)()
+            dm = (C.c_int * nl)()
+            off = _ll()
+            ctx.check(lib.su2_tensor_block(h, i, C.byref(idx), tj, dm, C.byref(off)))
+            self.blocks.append((idx.value, tuple(tj), tuple(dm), off.value))
+
+    def upload(self, flat):
+        flat = np.ascontiguousarray(flat, dtype=np.float64)
+        self.ctx.check(lib.su2_tensor_upload(self.ctx.h, self.h, flat.ctypes.data_as(_dp),
+                                             flat.size))
+
+    def download(self):
+        out = np.empty(self.size, dtype=np.float64)
+        self.ctx.check(lib.su2_tensor_download(self.ctx.h, self.h, out.ctypes.data_as(_dp),
+                                               out.size))
+        return out
+
+    def close(self):
+        if self.h:
+            lib.su2_tensor_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # Lanczos algebra
+    def copy_from(self, other):
+        self.ctx.check(lib.su2_tensor_copy(self.ctx.h, self.h, other.h))
+
+    def axpy(self, a, x):
+        self.ctx.check(lib.su2_tensor_axpy(self.ctx.h, self.h, float(a), x.h))
+
+    def scal(self, a):
+        self.ctx.check(lib.su2_tensor_scal(self.ctx.h, self.h, float(a)))
+
+    def dot(self, other):
+        r = _d()
+        self.ctx.check(lib.su2_tensor_dot(self.ctx.h, self.h, other.h, C.byref(r)))
+        return r.value
+
+    def normalize(self):
+        r = _d()
+        self.ctx.check(lib.su2_tensor_normalize(self.ctx.h, self.h, C.byref(r)))
+        return r.value
+
+
+# --------------------------------------------------------------------------
+# plain layout <-> flat block-index order
+# --------------------------------------------------------------------------
+def wavefunction(ctx, twoj, dims_a, dims_b, psi=None):
+    t = Tensor(ctx, [(twoj, dims_a, 1), (twoj, dims_b, 1)], locspin=0)
+    if psi is not None:
+        t.upload(pack_wavefunction(t, psi))
+    return t
+
+
+def pack_wavefunction(t, psi):
+    out = np.zeros(t.size)
+    for _, tj, dm, off in t.blocks:
+        blk = psi.get(tj[0])
+        if blk is not None:
+            out[off:off + dm[0] * dm[1]] = np.asarray(blk).ravel(order="F")
+    return out
+
+
+def unpack_wavefunction(t, flat):
+    return {tj[0]: flat[off:off + dm[0] * dm[1]].reshape((dm[0], dm[1]), order="F")
+            for _, tj, dm, off in t.blocks}
+
+
+def scalar_operator(ctx, twoj, dims, h):
+    t = Tensor(ctx, [(twoj, dims, 1), (twoj, dims, -1)], locspin=0)
+    flat = np.zeros(t.size)
+    for _, tj, dm, off in t.blocks:
+        if tj[0] in h:
+            flat[off:off + dm[0] * dm[1]] = np.asarray(h[tj[0]]).ravel(order="F")
+    t.upload(flat)
+    return t
+
+
+def vector_operator(ctx, twoj, dims, op):
+    """Three-leg [ket, spin-1 leg, bra] operator holding the blocks in ``op``."""
+    n = len(twoj)
+    present = np.zeros(n * n, dtype=np.int32)
+    pos = {j: i for i, j in enumerate(twoj)}
+    for (a, b) in op:
+        present[pos[a] + n * pos[b]] = 1
+    t = Tensor(ctx, [(twoj, dims, 1), ([2], [1], 1), (twoj, dims, -1)], locspin=0,
+               present=present)
+    flat = np.zeros(t.size)
+    for _, tj, dm, off in t.blocks:
+        flat[off:off + dm[0] * dm[2]] = np.asarray(op[(tj[0], tj[2])]).ravel(order="F")
+    t.upload(flat)
+    return t
+
+
+class HeffPlan:
+    """su2_heff: a built H_eff matvec plan.
+
+    ``terms`` = list of (left side, right side), each side a list of
+    (coef, Tensor) summed once at plan creation.
+    """
+
+    def __init__(self, ctx, psi_layout, hl, hr, terms):
+        self.ctx = ctx
+        lc, rc, lops, rops, lcoef, rcoef = [], [], [], [], [], []
+        for left, right in terms:
+            lc.append(len(left))
+            rc.append(len(right))
+            for c, t in left:
+                lcoef.append(c)
+                lops.append(t.h)
+            for c, t in right:
+                rcoef.append(c)
+                rops.append(t.h)
+        self._keep = (lops, rops)
+        a1, p1 = _ia(lc if lc else [0])
+        a2, p2 = _ia(rc if rc else [0])
+        LO = (_p * max(1, len(lops)))(*lops)
+        RO = (_p * max(1, len(rops)))(*rops)
+        LC = (C.c_double * max(1, len(lcoef)))(*lcoef)
+        RC = (C.c_double * max(1, len(rcoef)))(*rcoef)
+        h = _p()
+        ctx.check(lib.su2_heff_create(ctx.h, psi_layout.h, hl.h if hl else None,
+                                      hr.h if hr else None, len(terms), p1, LO, LC, p2, RO, RC,
+                                      C.byref(h)))
+        self.h = h
+
+    def apply(self, psi, out):
+        self.ctx.check(lib.su2_heff_apply(self.ctx.h, self.h, psi.h, out.h))
+
+    def flops(self):
+        a, e = _d(), _d()
+        self.ctx.check(lib.su2_heff_flops(self.h, C.byref(a), C.byref(e)))
+        return a.value, e.value
+
+    def time(self, psi, out, reps):
+        a, s1, s2 = _d(), _d(), _d()
+        self.ctx.check(lib.su2_heff_time(self.ctx.h, self.h, psi.h, out.h, int(reps), C.byref(a),
+                                         C.byref(s1), C.byref(s2)))
+        return a.value, s1.value, s2.value
+
+    def close(self):
+        if self.h:
+            lib.su2_heff_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def upload_workload(ctx, work):
+    """Device tensors + plan for a synthetic workload (see ``synthetic``)."""
+    twoj, da, db = work["twoj"], work["dims_a"], work["dims_b"]
+    psi = wavefunction(ctx, twoj, da, db, work["psi"])
+    out = wavefunction(ctx, twoj, da, db)
+    hl = scalar_operator(ctx, twoj, da, work["hl"]) if work.get("hl") is not None else None
+    hr = scalar_operator(ctx, twoj, db, work["hr"]) if work.get("hr") is not None else None
+    dev_terms = []
+    keep = []
+    for L, R in work["terms"]:
+        tl = vector_operator(ctx, twoj, da, L)
+        tr = vector_operator(ctx, twoj, db, R)
+        keep += [tl, tr]
+        dev_terms.append(([(1.0, tl)], [(1.0, tr)]))
+    plan = HeffPlan(ctx, psi, hl, hr, dev_terms)
+    return {"psi": psi, "out": out, "hl": hl, "hr": hr, "ops": keep, "plan": plan}

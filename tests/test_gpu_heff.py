@@ -1,0 +1,135 @@
+"""H_eff matvec on the MI355X through the C ABI versus the oracle (GPU)."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import heff as oheff
+from su2dmrg_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10   # relative 2-norm error bound for FP64 matvec parity
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from su2dmrg_amd import lib as L
+    c = L.Context(0)
+    yield c
+    c.close()
+
+
+def _relerr(got, ref):
+    num = sum(float(np.sum((got[j] - ref[j]) ** 2)) for j in ref)
+    den = sum(float(np.sum(ref[j] ** 2)) for j in ref)
+    return np.sqrt(num / den)
+
+
+def _run(ctx, work):
+    from su2dmrg_amd import lib as L
+    dev = L.upload_workload(ctx, work)
+    dev["plan"].apply(dev["psi"], dev["out"])
+    got = L.unpack_wavefunction(dev["out"], dev["out"].download())
+    return dev, got
+
+
+@pytest.mark.parametrize("seed,twoj,dims,nterm", [
+    (0, (0, 1, 2, 3, 4), (3, 4, 5, 2, 1), 3),
+    (1, (1, 3, 5), (17, 33, 9), 2),
+    (2, (0, 2, 4, 6), (130, 257, 129, 7), 4),
+    (3, (1, 3, 5, 7), (300, 161, 64, 1), 1),
+])
+def test_heff_small_matches_oracle(ctx, seed, twoj, dims, nterm):
+    _say(f"heff small seed={seed} twoj={twoj} dims={dims} nterm={nterm}")
+    work = synthetic.small(seed=seed, twoj=twoj, dims=dims, nterm=nterm)
+    _, got = _run(ctx, work)
+    terms = [(L, R) for L, R in work["terms"]]
+    ref = oheff.heff_apply(work["psi"], work["hl"], work["hr"], terms)
+    assert _relerr(got, ref) < RTOL
+
+
+def test_heff_cyl8x4_d1000_matches_oracle(ctx):
+    _say("heff cyl8x4_d1000 seed=0")
+    work = synthetic.cyl8x4_d1000(seed=0)
+    dev, got = _run(ctx, work)
+    ref = oheff.heff_apply(work["psi"], work["hl"], work["hr"], work["terms"])
+    assert _relerr(got, ref) < RTOL
+    alg, _ = dev["plan"].flops()
+    assert alg == synthetic.flops(work)
+
+
+def test_heff_without_block_hamiltonians_and_sparse_ops(ctx):
+    _say("heff sparse ops, no hl/hr")
+    work = synthetic.small(seed=11, twoj=(0, 2, 4, 6), dims=(20, 40, 30, 5), nterm=3)
+    # drop some transitions from some terms: the plan zero-fills the bundle
+    L0, R0 = work["terms"][0]
+    del L0[(2, 4)]
+    L1, R1 = work["terms"][1]
+    del R1[(4, 2)]
+    work["hl"] = None
+    work["hr"] = None
+    _, got = _run(ctx, work)
+    ref = oheff.heff_apply(work["psi"], None, None, work["terms"])
+    assert _relerr(got, ref) < RTOL
+
+
+def test_heff_presummed_sides(ctx):
+    from su2dmrg_amd import lib as L
+    _say("heff presum")
+    work = synthetic.small(seed=21, twoj=(1, 3, 5), dims=(40, 60, 20), nterm=3)
+    twoj, da, db = work["twoj"], work["dims_a"], work["dims_b"]
+    (L0, R0), (L1, R1), (L2, R2) = work["terms"]
+    psi = L.wavefunction(ctx, twoj, da, db, work["psi"])
+    out = L.wavefunction(ctx, twoj, da, db)
+    tl = [L.vector_operator(ctx, twoj, da, x) for x in (L0, L1)]
+    tr = [L.vector_operator(ctx, twoj, db, x) for x in (R0, R1, R2)]
+    plan = L.HeffPlan(ctx, psi, None, None,
+                      [([(1.0, tl[0])], [(0.5, tr[0]), (-2.0, tr[1])]),
+                       ([(0.3, tl[0]), (1.7, tl[1])], [(1.0, tr[2])])])
+    plan.apply(psi, out)
+    got = L.unpack_wavefunction(out, out.download())
+    terms = [(L0, oheff.presum([(0.5, R0), (-2.0, R1)])),
+             (oheff.presum([(0.3, L0), (1.7, L1)]), R2)]
+    ref = oheff.heff_apply(work["psi"], None, None, terms)
+    assert _relerr(got, ref) < RTOL
+
+
+def test_heff_deterministic(ctx):
+    from su2dmrg_amd import lib as L
+    _say("heff determinism")
+    work = synthetic.small(seed=5, twoj=(0, 2, 4), dims=(300, 500, 200), nterm=4)
+    dev = L.upload_workload(ctx, work)
+    dev["plan"].apply(dev["psi"], dev["out"])
+    a = dev["out"].download()
+    dev["plan"].apply(dev["psi"], dev["out"])
+    b = dev["out"].download()
+    assert np.array_equal(a, b)
+
+
+def test_vector_algebra(ctx):
+    from su2dmrg_amd import lib as L
+    _say("vector algebra")
+    work = synthetic.small(seed=9, twoj=(1, 3), dims=(70, 33), nterm=0)
+    twoj, da, db = work["twoj"], work["dims_a"], work["dims_b"]
+    rng = np.random.default_rng(3)
+    u = {j: rng.standard_normal(b.shape) for j, b in work["psi"].items()}
+    v = {j: rng.standard_normal(b.shape) for j, b in work["psi"].items()}
+    tu = L.wavefunction(ctx, twoj, da, db, u)
+    tv = L.wavefunction(ctx, twoj, da, db, v)
+    assert abs(tu.dot(tv) - oheff.inner(u, v)) < 1e-12 * oheff.norm(u) * oheff.norm(v)
+    tu.axpy(-0.25, tv)
+    ref = oheff.axpy(-0.25, v, u)
+    got = L.unpack_wavefunction(tu, tu.download())
+    assert _relerr(got, ref) < 1e-14
+    n = tu.normalize()
+    assert abs(n - oheff.norm(ref)) < 1e-12 * n
+    tw = L.wavefunction(ctx, twoj, da, db)
+    tw.copy_from(tu)
+    tw.scal(3.0)
+    assert abs(tw.dot(tw) - 9.0) < 1e-12
