@@ -82,6 +82,10 @@ def main():
     assert abs(alg - flops) < 1e-6 * flops
     log(f"plan built: {alg / 1e9:.1f} GFLOP algorithmic, {executed / 1e9:.1f} GFLOP executed")
 
+    plan.apply(dev["psi"], dev["out"])
+    chosen, t1, t2 = plan.autotune(reps=3)
+    log(f"autotune: stage1 variant {chosen[0]} ({' '.join(f'{t:.2f}' for t in t1)} ms), "
+        f"stage2 variant {chosen[1]} ({' '.join(f'{t:.2f}' for t in t2)} ms)")
     for _ in range(args.warmup):
         plan.apply(dev["psi"], dev["out"])
     ctx.sync()
@@ -126,6 +130,7 @@ def main():
                         f"sectors 2j={work['twoj']} dims={work['dims_a']}",
             "algorithmic_gflop_per_step": round(alg / 1e9, 3),
             "sweep_time_s": None,
+            "gemm_variants": list(chosen),
         },
         "roofline": {
             "bound": "mfma",

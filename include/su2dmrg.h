@@ -110,6 +110,17 @@ int su2_heff_destroy(su2_ctx* ctx, su2_heff* plan);
 int su2_heff_time(su2_ctx* ctx, su2_heff* plan, const su2_tensor* psi, su2_tensor* out, int reps,
                   double* ms_apply, double* ms_stage1, double* ms_stage2);
 
+/* Tile-shape autotuner of the two grouped FP64 MFMA launches of a plan: every
+ * compiled tile variant is timed (HIP events, `reps` launches) on the plan's
+ * own problem list and the fastest is kept.  chosen[2] receives the variant of
+ * stage 1 and stage 2; times (nullable) receives 2*nvariants ms per launch
+ * (stage 1 first).  *nvariants receives the number of compiled variants.
+ * No counterpart in the reference (which calls MKL dgemm per block). */
+int su2_heff_autotune(su2_ctx* ctx, su2_heff* plan, int reps, int* chosen, float* times,
+                      int* nvariants);
+/* Pin the tile variants of the two launches (e.g. to a tuned choice). */
+int su2_heff_set_variants(su2_ctx* ctx, su2_heff* plan, int stage1, int stage2);
+
 #ifdef __cplusplus
 }
 #endif

@@ -345,4 +345,36 @@ int su2_heff_time(su2_ctx* ctx, su2_heff* plan, const su2_tensor* psi, su2_tenso
   });
 }
 
+int su2_heff_autotune(su2_ctx* ctx, su2_heff* plan, int reps, int* chosen, float* times,
+                      int* nvariants) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(plan, "plan");
+    SU2_CHECK(reps >= 1, su2::kErrArgument, "reps=%d", reps);
+    std::vector<float> t1, t2;
+    int ch[2];
+    plan->p.autotune(reps, ch, &t1, &t2);
+    if (chosen) {
+      chosen[0] = ch[0];
+      chosen[1] = ch[1];
+    }
+    const int nv = su2::gemm_num_variants();
+    if (nvariants) *nvariants = nv;
+    if (times) {
+      for (int v = 0; v < nv; ++v) {
+        times[v] = t1[v];
+        times[nv + v] = t2[v];
+      }
+    }
+  });
+}
+
+int su2_heff_set_variants(su2_ctx* ctx, su2_heff* plan, int stage1, int stage2) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(plan, "plan");
+    plan->p.set_variants(stage1, stage2);
+  });
+}
+
 }  // extern "C"

@@ -24,19 +24,6 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
-constexpr int BM = kGemmBM;
-constexpr int BN = kGemmBN;
-constexpr int BK = kGemmBK;
-constexpr int WAVES_M = 2;
-constexpr int WAVES_N = 2;
-constexpr int NT = WAVES_M * WAVES_N * 64;
-constexpr int WTM = BM / WAVES_M;  // 64
-constexpr int WTN = BN / WAVES_N;  // 64
-constexpr int FM = WTM / 16;       // 4
-constexpr int FN = WTN / 16;       // 4
-constexpr int A_PER_T = BM * BK / NT;  // 8
-constexpr int B_PER_T = BN * BK / NT;  // 8
-
 // Logical work-unit id of this workgroup: the hardware deals workgroups to
 // the 8 XCDs round-robin (id % 8); give each XCD a contiguous run of logical
 // units so neighbouring tiles (which share A or B panels) meet in one L2.
@@ -57,92 +44,129 @@ __device__ __forceinline__ int find_problem(const GemmProblem* __restrict__ prob
   return lo;
 }
 
-struct SlabRegs {
-  double a[A_PER_T];
-  double b[B_PER_T];
+// Global-address-space view of a device pointer: loads through it compile to
+// global_load (vmcnt only) instead of flat_load, whose lgkmcnt would make every
+// LDS wait in the MFMA loop also wait for the next slab's memory traffic.
+typedef const double __attribute__((address_space(1)))* gdptr;
+__device__ __forceinline__ double gload(const double* p, long long i) { return ((gdptr)p)[i]; }
+
+template <int BM, int BN, int BK, int WM, int WN, int PAD>
+struct Tile {
+  static constexpr int NT = WM * WN * 64;
+  static constexpr int WTM = BM / WM;
+  static constexpr int WTN = BN / WN;
+  static constexpr int FM = WTM / 16;
+  static constexpr int FN = WTN / 16;
+  static constexpr int APT = BM * BK / NT;   // A elements per thread per slab
+  static constexpr int BPT = BN * BK / NT;   // B elements per thread per slab
+  static constexpr int LDA_S = BM + PAD;     // LDS row pitch (doubles)
+  static constexpr int LDB_S = BN + PAD;
+  static_assert(BM * BK % NT == 0 && BN * BK % NT == 0, "slab must split evenly");
+  static_assert(WTM % 16 == 0 && WTN % 16 == 0 && BK % 4 == 0, "MFMA tiling");
+  static_assert(APT <= 32 && BPT <= 32, "mask width");
 };
 
+template <class T>
+struct SlabRegs {
+  double a[T::APT];
+  double b[T::BPT];
+  unsigned amask, bmask;   // bit i set: element i lies inside the matrix
+};
+
+// Issue the global loads of one BK slab.  Addresses are clamped into the
+// matrix so every load is unconditional (a branch per load would make the
+// compiler wait for each one); out-of-range elements are zeroed by the mask
+// in store_slab, after the matrix cores have consumed the previous slab.
+template <int BM, int BN, int BK, int WM, int WN, int PAD>
 __device__ __forceinline__ void load_slab(const GemmSlice& s, int kt_local, int m0, int n0, int M,
-                                          int N, SlabRegs& r) {
+                                          int N, SlabRegs<Tile<BM, BN, BK, WM, WN, PAD>>& r) {
+  using T = Tile<BM, BN, BK, WM, WN, PAD>;
   const int t = threadIdx.x;
   const int k0 = kt_local * BK;
-  const double alpha = s.alpha;
+  r.amask = 0;
+  r.bmask = 0;
   if (!s.transA) {
     // op(A) = A, column-major M x K: coalesced along m.
 #pragma unroll
-    for (int i = 0; i < A_PER_T; ++i) {
-      int e = t + i * NT;
-      int m = e % BM, k = e / BM;
-      int gm = m0 + m, gk = k0 + k;
-      r.a[i] = (gm < M && gk < s.K) ? alpha * s.A[gm + (long long)gk * s.lda] : 0.0;
+    for (int i = 0; i < T::APT; ++i) {
+      int e = t + i * T::NT;
+      int gm = m0 + e % BM, gk = k0 + e / BM;
+      r.amask |= (unsigned)(gm < M && gk < s.K) << i;
+      r.a[i] = gload(s.A, min(gm, M - 1) + (long long)min(gk, s.K - 1) * s.lda);
     }
   } else {
     // op(A) = A^T, A stored K x M: coalesced along k.
 #pragma unroll
-    for (int i = 0; i < A_PER_T; ++i) {
-      int e = t + i * NT;
-      int k = e % BK, m = e / BK;
-      int gm = m0 + m, gk = k0 + k;
-      r.a[i] = (gm < M && gk < s.K) ? alpha * s.A[gk + (long long)gm * s.lda] : 0.0;
+    for (int i = 0; i < T::APT; ++i) {
+      int e = t + i * T::NT;
+      int gk = k0 + e % BK, gm = m0 + e / BK;
+      r.amask |= (unsigned)(gm < M && gk < s.K) << i;
+      r.a[i] = gload(s.A, min(gk, s.K - 1) + (long long)min(gm, M - 1) * s.lda);
     }
   }
   if (!s.transB) {
     // B stored K x N: coalesced along k.
 #pragma unroll
-    for (int i = 0; i < B_PER_T; ++i) {
-      int e = t + i * NT;
-      int k = e % BK, n = e / BK;
-      int gn = n0 + n, gk = k0 + k;
-      r.b[i] = (gn < N && gk < s.K) ? s.B[gk + (long long)gn * s.ldb] : 0.0;
+    for (int i = 0; i < T::BPT; ++i) {
+      int e = t + i * T::NT;
+      int gk = k0 + e % BK, gn = n0 + e / BK;
+      r.bmask |= (unsigned)(gn < N && gk < s.K) << i;
+      r.b[i] = gload(s.B, min(gk, s.K - 1) + (long long)min(gn, N - 1) * s.ldb);
     }
   } else {
     // op(B) = B^T, B stored N x K: coalesced along n.
 #pragma unroll
-    for (int i = 0; i < B_PER_T; ++i) {
-      int e = t + i * NT;
-      int n = e % BN, k = e / BN;
-      int gn = n0 + n, gk = k0 + k;
-      r.b[i] = (gn < N && gk < s.K) ? s.B[gn + (long long)gk * s.ldb] : 0.0;
+    for (int i = 0; i < T::BPT; ++i) {
+      int e = t + i * T::NT;
+      int gn = n0 + e % BN, gk = k0 + e / BN;
+      r.bmask |= (unsigned)(gn < N && gk < s.K) << i;
+      r.b[i] = gload(s.B, min(gn, N - 1) + (long long)min(gk, s.K - 1) * s.ldb);
     }
   }
 }
 
-__device__ __forceinline__ void store_slab(const GemmSlice& s, const SlabRegs& r,
-                                           double (*As)[BM], double (*Bs)[BN]) {
+template <int BM, int BN, int BK, int WM, int WN, int PAD>
+__device__ __forceinline__ void store_slab(const GemmSlice& s,
+                                           const SlabRegs<Tile<BM, BN, BK, WM, WN, PAD>>& r,
+                                           double* As, double* Bs) {
+  using T = Tile<BM, BN, BK, WM, WN, PAD>;
   const int t = threadIdx.x;
+  const double alpha = s.alpha;
   if (!s.transA) {
 #pragma unroll
-    for (int i = 0; i < A_PER_T; ++i) {
-      int e = t + i * NT;
-      As[e / BM][e % BM] = r.a[i];
+    for (int i = 0; i < T::APT; ++i) {
+      int e = t + i * T::NT;
+      As[(e / BM) * T::LDA_S + e % BM] = ((r.amask >> i) & 1u) ? alpha * r.a[i] : 0.0;
     }
   } else {
 #pragma unroll
-    for (int i = 0; i < A_PER_T; ++i) {
-      int e = t + i * NT;
-      As[e % BK][e / BK] = r.a[i];
+    for (int i = 0; i < T::APT; ++i) {
+      int e = t + i * T::NT;
+      As[(e % BK) * T::LDA_S + e / BK] = ((r.amask >> i) & 1u) ? alpha * r.a[i] : 0.0;
     }
   }
   if (!s.transB) {
 #pragma unroll
-    for (int i = 0; i < B_PER_T; ++i) {
-      int e = t + i * NT;
-      Bs[e % BK][e / BK] = r.b[i];
+    for (int i = 0; i < T::BPT; ++i) {
+      int e = t + i * T::NT;
+      Bs[(e % BK) * T::LDB_S + e / BK] = ((r.bmask >> i) & 1u) ? r.b[i] : 0.0;
     }
   } else {
 #pragma unroll
-    for (int i = 0; i < B_PER_T; ++i) {
-      int e = t + i * NT;
-      Bs[e / BN][e % BN] = r.b[i];
+    for (int i = 0; i < T::BPT; ++i) {
+      int e = t + i * T::NT;
+      Bs[(e / BN) * T::LDB_S + e % BN] = ((r.bmask >> i) & 1u) ? r.b[i] : 0.0;
     }
   }
 }
 
-__global__ __launch_bounds__(NT) void grouped_gemm_f64_kernel(
+template <int BM, int BN, int BK, int WM, int WN, int PAD>
+__global__ __launch_bounds__(WM * WN * 64) void grouped_gemm_f64_kernel(
     const GemmProblem* __restrict__ probs, int nprob, const GemmSlice* __restrict__ slices,
     double* __restrict__ ws, long long total_units) {
-  __shared__ double As[2][BK][BM];
-  __shared__ double Bs[2][BK][BN];
+  using T = Tile<BM, BN, BK, WM, WN, PAD>;
+  __shared__ double As[2][BK * T::LDA_S];
+  __shared__ double Bs[2][BK * T::LDB_S];
 
   const long long unit = xcd_remap(blockIdx.x, gridDim.x);
   if (unit >= total_units) return;
@@ -160,14 +184,14 @@ __global__ __launch_bounds__(NT) void grouped_gemm_f64_kernel(
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int wm0 = (wave % WAVES_M) * WTM;
-  const int wn0 = (wave / WAVES_M) * WTN;
+  const int wm0 = (wave % WM) * T::WTM;
+  const int wn0 = (wave / WM) * T::WTN;
 
-  f64x4 acc[FM][FN];
+  f64x4 acc[T::FM][T::FN];
 #pragma unroll
-  for (int i = 0; i < FM; ++i)
+  for (int i = 0; i < T::FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < T::FN; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
 
   if (kt_lo < kt_hi) {
     // locate the slice holding k-tile kt_lo
@@ -176,9 +200,9 @@ __global__ __launch_bounds__(NT) void grouped_gemm_f64_kernel(
     GemmSlice S = slices[s];
     int kl = kt_lo - S.kt_begin;
 
-    SlabRegs regs;
-    load_slab(S, kl, m0, n0, P.M, P.N, regs);
-    store_slab(S, regs, As[0], Bs[0]);
+    SlabRegs<T> regs;
+    load_slab<BM, BN, BK, WM, WN, PAD>(S, kl, m0, n0, P.M, P.N, regs);
+    store_slab<BM, BN, BK, WM, WN, PAD>(S, regs, As[0], Bs[0]);
     __syncthreads();
 
     int buf = 0;
@@ -194,23 +218,26 @@ __global__ __launch_bounds__(NT) void grouped_gemm_f64_kernel(
           Sn = slices[s];
           kln = 0;
         }
-        load_slab(Sn, kln, m0, n0, P.M, P.N, regs);
+        load_slab<BM, BN, BK, WM, WN, PAD>(Sn, kln, m0, n0, P.M, P.N, regs);
       }
+      const double* Ab = As[buf];
+      const double* Bb = Bs[buf];
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
-        double af[FM], bf[FN];
+        double af[T::FM], bf[T::FN];
+        const int kr = kk + (lane >> 4);
 #pragma unroll
-        for (int i = 0; i < FM; ++i) af[i] = As[buf][kk + (lane >> 4)][wm0 + i * 16 + (lane & 15)];
+        for (int i = 0; i < T::FM; ++i) af[i] = Ab[kr * T::LDA_S + wm0 + i * 16 + (lane & 15)];
 #pragma unroll
-        for (int j = 0; j < FN; ++j) bf[j] = Bs[buf][kk + (lane >> 4)][wn0 + j * 16 + (lane & 15)];
+        for (int j = 0; j < T::FN; ++j) bf[j] = Bb[kr * T::LDB_S + wn0 + j * 16 + (lane & 15)];
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+        for (int i = 0; i < T::FM; ++i)
 #pragma unroll
-          for (int j = 0; j < FN; ++j)
+          for (int j = 0; j < T::FN; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
       }
       if (more) {
-        store_slab(Sn, regs, As[buf ^ 1], Bs[buf ^ 1]);
+        store_slab<BM, BN, BK, WM, WN, PAD>(Sn, regs, As[buf ^ 1], Bs[buf ^ 1]);
         S = Sn;
         kl = kln;
       }
@@ -230,9 +257,9 @@ __global__ __launch_bounds__(NT) void grouped_gemm_f64_kernel(
     ld = P.M;
   }
 #pragma unroll
-  for (int i = 0; i < FM; ++i)
+  for (int i = 0; i < T::FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j)
+    for (int j = 0; j < T::FN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         int row = m0 + wm0 + i * 16 + (lane >> 4) + 4 * r;
@@ -261,17 +288,50 @@ __global__ void splitk_reduce_kernel(const ReduceProblem* __restrict__ reds, int
   R.C[row + (long long)col * R.ldc] = s;
 }
 
+typedef void (*GemmKernel)(const GemmProblem*, int, const GemmSlice*, double*, long long);
+
+struct VariantInfo {
+  GemmKernel fn;
+  int bm, bn, bk, threads;
+};
+
+#define SU2_VARIANT(BM, BN, BK, WM, WN, PAD) \
+  {grouped_gemm_f64_kernel<BM, BN, BK, WM, WN, PAD>, BM, BN, BK, WM * WN * 64}
+
+const VariantInfo kVariants[] = {
+    SU2_VARIANT(128, 128, 16, 2, 2, 0),   // 0
+    SU2_VARIANT(128, 128, 16, 2, 2, 2),   // 1: padded LDS rows
+    SU2_VARIANT(256, 128, 16, 4, 2, 2),   // 2
+    SU2_VARIANT(128, 256, 16, 2, 4, 2),   // 3
+    SU2_VARIANT(64, 64, 16, 2, 2, 2),     // 4
+    SU2_VARIANT(128, 128, 32, 2, 2, 2),   // 5
+    SU2_VARIANT(128, 64, 16, 2, 2, 2),    // 6
+};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
 }  // namespace
 
-GroupedGemm::~GroupedGemm() {
+int gemm_num_variants() { return kNumVariants; }
+
+GroupedGemm::~GroupedGemm() { release(); }
+
+void GroupedGemm::release() {
   if (d_probs_) (void)hipFree(d_probs_);
   if (d_slices_) (void)hipFree(d_slices_);
   if (d_reds_) (void)hipFree(d_reds_);
   if (d_ws_) (void)hipFree(d_ws_);
+  d_probs_ = nullptr;
+  d_slices_ = nullptr;
+  d_reds_ = nullptr;
+  d_ws_ = nullptr;
+  run_probs_.clear();
+  reds_.clear();
+  total_units_ = 0;
+  total_red_elems_ = 0;
 }
 
 int GroupedGemm::add_problem(double* C, long long ldc, int M, int N) {
-  SU2_CHECK(!finalized_, kErrArgument, "gemm plan already finalized");
+  SU2_CHECK(variant_ < 0, kErrArgument, "gemm plan already finalized");
   SU2_CHECK(M >= 0 && N >= 0 && ldc >= std::max(M, 1), kErrArgument, "gemm: bad problem shape");
   GemmProblem p{};
   p.C = C;
@@ -285,48 +345,58 @@ int GroupedGemm::add_problem(double* C, long long ldc, int M, int N) {
 
 void GroupedGemm::add_slice(int problem, const double* A, long long lda, int transA,
                             const double* B, long long ldb, int transB, int K, double alpha) {
+  SU2_CHECK(variant_ < 0, kErrArgument, "gemm plan already finalized");
   SU2_CHECK(problem == (int)probs_.size() - 1, kErrArgument,
             "gemm: slices must follow their problem");
   GemmProblem& p = probs_[problem];
   SU2_CHECK(K > 0, kErrArgument, "gemm: empty slice");
   SU2_CHECK(lda >= (transA ? K : std::max(p.M, 1)) && ldb >= (transB ? std::max(p.N, 1) : K),
             kErrArgument, "gemm: leading dimension too small");
-  GemmSlice s{};
-  s.A = A;
-  s.B = B;
-  s.lda = lda;
-  s.ldb = ldb;
-  s.K = K;
-  s.transA = transA;
-  s.transB = transB;
-  s.alpha = alpha;
-  slices_.push_back(s);
+  GemmSlice sl{};
+  sl.A = A;
+  sl.B = B;
+  sl.lda = lda;
+  sl.ldb = ldb;
+  sl.K = K;
+  sl.transA = transA;
+  sl.transB = transB;
+  sl.alpha = alpha;
+  slices_.push_back(sl);
   p.slice_end = (int)slices_.size();
   flops_ += 2.0 * p.M * (double)p.N * K;
 }
 
-void GroupedGemm::finalize(int target_units) {
-  SU2_CHECK(!finalized_, kErrArgument, "gemm plan already finalized");
-  // drop empty problems (M or N zero); problems without slices are zero-filled
-  // by the caller before launch, they are rejected here.
-  double work = 0;
-  for (auto& p : probs_) {
+void GroupedGemm::finalize(int target_units, int variant) {
+  SU2_CHECK(variant_ < 0, kErrArgument, "gemm plan already finalized");
+  for (auto& p : probs_)
     SU2_CHECK(p.slice_end > p.slice_begin, kErrArgument, "gemm: problem without K slices");
+  target_units_ = std::max(1, target_units);
+  set_variant(variant < 0 ? kDefaultGemmVariant : variant);
+}
+
+void GroupedGemm::set_variant(int v) {
+  SU2_CHECK(v >= 0 && v < kNumVariants, kErrArgument, "gemm variant %d out of range", v);
+  const VariantInfo& V = kVariants[v];
+  release();
+  variant_ = v;
+  std::vector<GemmSlice> sl = slices_;
+  double work = 0;
+  std::vector<GemmProblem> probs = probs_;
+  for (auto& p : probs) {
     int kt = 0;
     for (int s = p.slice_begin; s < p.slice_end; ++s) {
-      slices_[s].kt_begin = kt;
-      kt += (slices_[s].K + BK - 1) / BK;
+      sl[s].kt_begin = kt;
+      kt += (sl[s].K + V.bk - 1) / V.bk;
     }
     p.kt_total = kt;
-    p.tiles_m = (p.M + BM - 1) / BM;
-    p.tiles_n = (p.N + BN - 1) / BN;
+    p.tiles_m = (p.M + V.bm - 1) / V.bm;
+    p.tiles_n = (p.N + V.bn - 1) / V.bn;
     work += (double)p.tiles_m * p.tiles_n * kt;
   }
   // split K so that the launch has about target_units comparable work units
-  double unit_work = std::max(1.0, work / std::max(1, target_units));
+  const double unit_work = std::max(1.0, work / target_units_);
   long long units = 0, ws = 0;
-  std::vector<GemmProblem> kept;
-  for (auto& p : probs_) {
+  for (auto& p : probs) {
     if (p.M == 0 || p.N == 0) continue;
     int ks = (int)std::floor((double)p.kt_total / unit_work);
     ks = std::max(1, std::min(ks, std::max(1, p.kt_total / 8)));
@@ -347,18 +417,18 @@ void GroupedGemm::finalize(int target_units) {
       reds_.push_back(r);
       ws += (long long)ks * p.M * p.N;
     }
-    kept.push_back(p);
+    run_probs_.push_back(p);
   }
-  probs_.swap(kept);
   total_units_ = units;
-  if (!probs_.empty()) {
-    SU2_HIP(hipMalloc(&d_probs_, probs_.size() * sizeof(GemmProblem)));
-    SU2_HIP(hipMemcpy(d_probs_, probs_.data(), probs_.size() * sizeof(GemmProblem),
+  SU2_CHECK(total_units_ < (1ll << 31), kErrCapacity, "gemm: too many work units");
+  if (!run_probs_.empty()) {
+    SU2_HIP(hipMalloc(&d_probs_, run_probs_.size() * sizeof(GemmProblem)));
+    SU2_HIP(hipMemcpy(d_probs_, run_probs_.data(), run_probs_.size() * sizeof(GemmProblem),
                       hipMemcpyHostToDevice));
   }
-  if (!slices_.empty()) {
-    SU2_HIP(hipMalloc(&d_slices_, slices_.size() * sizeof(GemmSlice)));
-    SU2_HIP(hipMemcpy(d_slices_, slices_.data(), slices_.size() * sizeof(GemmSlice),
+  if (!sl.empty()) {
+    SU2_HIP(hipMalloc(&d_slices_, sl.size() * sizeof(GemmSlice)));
+    SU2_HIP(hipMemcpy(d_slices_, sl.data(), sl.size() * sizeof(GemmSlice),
                       hipMemcpyHostToDevice));
   }
   if (!reds_.empty()) {
@@ -367,15 +437,14 @@ void GroupedGemm::finalize(int target_units) {
                       hipMemcpyHostToDevice));
     SU2_HIP(hipMalloc(&d_ws_, ws * sizeof(double)));
   }
-  finalized_ = true;
 }
 
 void GroupedGemm::launch(hipStream_t stream) const {
-  SU2_CHECK(finalized_, kErrArgument, "gemm plan not finalized");
+  SU2_CHECK(variant_ >= 0, kErrArgument, "gemm plan not finalized");
   if (total_units_ == 0) return;
-  SU2_CHECK(total_units_ < (1ll << 31), kErrCapacity, "gemm: too many work units");
-  hipLaunchKernelGGL(grouped_gemm_f64_kernel, dim3((unsigned)total_units_), dim3(NT), 0, stream,
-                     d_probs_, (int)probs_.size(), d_slices_, d_ws_, total_units_);
+  const VariantInfo& V = kVariants[variant_];
+  hipLaunchKernelGGL(V.fn, dim3((unsigned)total_units_), dim3(V.threads), 0, stream, d_probs_,
+                     (int)run_probs_.size(), d_slices_, d_ws_, total_units_);
   SU2_HIP(hipGetLastError());
   if (!reds_.empty()) {
     long long blocks = (total_red_elems_ + 255) / 256;
@@ -383,6 +452,40 @@ void GroupedGemm::launch(hipStream_t stream) const {
                        d_reds_, (int)reds_.size(), d_ws_, total_red_elems_);
     SU2_HIP(hipGetLastError());
   }
+}
+
+float GroupedGemm::time_ms(hipStream_t stream, int reps) const {
+  hipEvent_t e0, e1;
+  SU2_HIP(hipEventCreate(&e0));
+  SU2_HIP(hipEventCreate(&e1));
+  launch(stream);  // warm
+  SU2_HIP(hipEventRecord(e0, stream));
+  for (int r = 0; r < reps; ++r) launch(stream);
+  SU2_HIP(hipEventRecord(e1, stream));
+  SU2_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  SU2_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms / reps;
+}
+
+int GroupedGemm::autotune(hipStream_t stream, int reps, std::vector<float>* times) {
+  SU2_CHECK(variant_ >= 0, kErrArgument, "gemm plan not finalized");
+  int best = variant_;
+  float best_ms = 1e30f;
+  if (times) times->assign(kNumVariants, 0.f);
+  for (int v = 0; v < kNumVariants; ++v) {
+    set_variant(v);
+    float ms = time_ms(stream, reps);
+    if (times) (*times)[v] = ms;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = v;
+    }
+  }
+  set_variant(best);
+  return best;
 }
 
 }  // namespace su2

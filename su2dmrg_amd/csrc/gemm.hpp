@@ -46,7 +46,11 @@ struct ReduceProblem {
   long long elem_begin;
 };
 
-// Host-side plan: build problems, then upload once; launch many times.
+// Number of compiled tile variants (BM x BN x BK x waves) and the default.
+int gemm_num_variants();
+constexpr int kDefaultGemmVariant = 1;
+
+// Host-side plan: build problems, then finalize once; launch many times.
 class GroupedGemm {
  public:
   GroupedGemm() = default;
@@ -58,8 +62,16 @@ class GroupedGemm {
   int add_problem(double* C, long long ldc, int M, int N);
   void add_slice(int problem, const double* A, long long lda, int transA,
                  const double* B, long long ldb, int transB, int K, double alpha);
-  // Chooses K splits for a target number of work units and uploads tables.
-  void finalize(int target_units);
+  // Chooses K splits for about target_units work units with tile variant
+  // `variant` (-1: default) and uploads the tables.
+  void finalize(int target_units, int variant = -1);
+  // Re-tiles for another variant (device tables rebuilt).
+  void set_variant(int v);
+  int variant() const { return variant_; }
+  // Times every variant (HIP events, `reps` launches each) and keeps the
+  // fastest; returns it.  `times` receives ms per launch by variant.
+  int autotune(hipStream_t stream, int reps, std::vector<float>* times = nullptr);
+  float time_ms(hipStream_t stream, int reps) const;
   void launch(hipStream_t stream) const;
 
   double flops() const { return flops_; }
@@ -67,8 +79,10 @@ class GroupedGemm {
   size_t nproblems() const { return probs_.size(); }
 
  private:
-  std::vector<GemmProblem> probs_;
+  void release();
+  std::vector<GemmProblem> probs_;      // as added
   std::vector<GemmSlice> slices_;
+  std::vector<GemmProblem> run_probs_;  // tiled for the current variant
   std::vector<ReduceProblem> reds_;
   GemmProblem* d_probs_ = nullptr;
   GemmSlice* d_slices_ = nullptr;
@@ -77,12 +91,8 @@ class GroupedGemm {
   long long total_units_ = 0;
   long long total_red_elems_ = 0;
   double flops_ = 0;
-  bool finalized_ = false;
+  int target_units_ = 2048;
+  int variant_ = -1;
 };
-
-// Tile shape of the FP64 MFMA kernel.
-constexpr int kGemmBM = 128;
-constexpr int kGemmBN = 128;
-constexpr int kGemmBK = 16;
 
 }  // namespace su2

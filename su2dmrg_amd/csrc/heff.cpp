@@ -248,6 +248,25 @@ void HeffPlan::apply_raw(const double* psi, double* out) {
   dev_copy(ctx_, out, d_out_, psi_size_);
 }
 
+void HeffPlan::set_variants(int v1, int v2) {
+  ctx_.make_current();
+  stage1_.set_variant(v1);
+  stage2_.set_variant(v2);
+}
+
+void HeffPlan::autotune(int reps, int* chosen, std::vector<float>* t1, std::vector<float>* t2) {
+  ctx_.make_current();
+  // stage 2 reads the stage-1 output; run once so its inputs are defined
+  stage1_.launch(ctx_.stream());
+  int a = stage1_.autotune(ctx_.stream(), reps, t1);
+  int b = stage2_.autotune(ctx_.stream(), reps, t2);
+  SU2_HIP(hipStreamSynchronize(ctx_.stream()));
+  if (chosen) {
+    chosen[0] = a;
+    chosen[1] = b;
+  }
+}
+
 void HeffPlan::apply(const Tensor& psi, Tensor& out) {
   SU2_CHECK(psi.size() == psi_size_ && out.size() == psi_size_, kErrArgument,
             "H_eff apply: vector layout differs from the plan");

@@ -49,6 +49,10 @@ class HeffPlan {
   long long psi_size() const { return psi_size_; }
   const GroupedGemm& stage1() const { return stage1_; }
   const GroupedGemm& stage2() const { return stage2_; }
+  // Tile-variant choice of the two grouped launches: set explicitly, or time
+  // every compiled variant on this plan's own problems and keep the fastest.
+  void set_variants(int v1, int v2);
+  void autotune(int reps, int* chosen, std::vector<float>* t1, std::vector<float>* t2);
 
  private:
   Context& ctx_;

@@ -54,6 +54,8 @@ _SIG = {
     "su2_heff_flops": (_i, [_p, _dp, _dp]),
     "su2_heff_destroy": (_i, [_p, _p]),
     "su2_heff_time": (_i, [_p, _p, _p, _p, _i, _dp, _dp, _dp]),
+    "su2_heff_autotune": (_i, [_p, _p, _i, _ip, C.POINTER(C.c_float), _ip]),
+    "su2_heff_set_variants": (_i, [_p, _p, _i, _i]),
 }
 
 for _name, (_res, _args) in _SIG.items():
@@ -284,6 +286,19 @@ class HeffPlan:
         self.ctx.check(lib.su2_heff_time(self.ctx.h, self.h, psi.h, out.h, int(reps), C.byref(a),
                                          C.byref(s1), C.byref(s2)))
         return a.value, s1.value, s2.value
+
+    def autotune(self, reps=3):
+        """Time every tile variant for both launches; returns (chosen, times1, times2)."""
+        ch = (C.c_int * 2)()
+        nv = C.c_int()
+        times = (C.c_float * 64)()
+        self.ctx.check(lib.su2_heff_autotune(self.ctx.h, self.h, int(reps), ch, times,
+                                             C.byref(nv)))
+        n = nv.value
+        return (ch[0], ch[1]), list(times[:n]), list(times[n:2 * n])
+
+    def set_variants(self, v1, v2):
+        self.ctx.check(lib.su2_heff_set_variants(self.ctx.h, self.h, int(v1), int(v2)))
 
     def close(self):
         if self.h:

@@ -112,6 +112,22 @@ def test_heff_deterministic(ctx):
     assert np.array_equal(a, b)
 
 
+def test_every_tile_variant_matches_oracle(ctx):
+    from su2dmrg_amd import lib as L
+    _say("heff tile variants")
+    work = synthetic.small(seed=13, twoj=(1, 3, 5, 7), dims=(150, 290, 77, 3), nterm=3)
+    dev = L.upload_workload(ctx, work)
+    ref = oheff.heff_apply(work["psi"], work["hl"], work["hr"], work["terms"])
+    chosen, t1, t2 = dev["plan"].autotune(reps=1)
+    assert len(t1) == len(t2) >= 2 and all(t > 0 for t in t1 + t2)
+    for v in range(len(t1)):
+        _say(f"  variant {v}")
+        dev["plan"].set_variants(v, v)
+        dev["plan"].apply(dev["psi"], dev["out"])
+        got = L.unpack_wavefunction(dev["out"], dev["out"].download())
+        assert _relerr(got, ref) < RTOL, v
+
+
 def test_vector_algebra(ctx):
     from su2dmrg_amd import lib as L
     _say("vector algebra")
