@@ -1,0 +1,77 @@
+"""Exact diagonalisation of the same spin-1/2 Hamiltonian (oracle truth anchor).
+
+TEST INFRASTRUCTURE ONLY -- see ``oracle/__init__.py``.
+
+H = sum_{i<j} J_ij S_i . S_j with J_ij = coup[hmap[i][j]] from the oracle's
+Lattice (dmrg_su2.cpp:97-260), built in the S_z = M sector of the 2^ns
+product basis; the lowest level of total spin S is found by diagonalising
+H + lambda * (S^2 - S(S+1))^2-free shift: we take the lowest eigenvalues in
+S_z = S and keep the first whose <S^2> equals S(S+1).
+"""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+
+def _basis(ns, twoM):
+    nup = (ns + twoM) // 2
+    states = []
+    for ups in itertools.combinations(range(ns), nup):
+        s = 0
+        for u in ups:
+            s |= 1 << u
+        states.append(s)
+    states.sort()
+    return np.array(states, dtype=np.int64)
+
+
+def _heisenberg(ns, bonds, states):
+    index = {int(s): i for i, s in enumerate(states)}
+    rows, cols, vals = [], [], []
+    diag = np.zeros(len(states))
+    for a, s in enumerate(states):
+        s = int(s)
+        for (i, j, J) in bonds:
+            si = (s >> i) & 1
+            sj = (s >> j) & 1
+            if si == sj:
+                diag[a] += 0.25 * J
+            else:
+                diag[a] -= 0.25 * J
+                t = s ^ ((1 << i) | (1 << j))
+                rows.append(a)
+                cols.append(index[t])
+                vals.append(0.5 * J)
+    n = len(states)
+    H = sp.csr_matrix((vals, (rows, cols)), shape=(n, n)) + sp.diags(diag)
+    return H
+
+
+def _s2(ns, states, vec):
+    # <S^2> = sum_ij <S_i . S_j>
+    bonds = [(i, j, 2.0) for i in range(ns) for j in range(i + 1, ns)]
+    H = _heisenberg(ns, bonds, states)
+    return float(vec @ (H @ vec)) + 0.75 * ns
+
+
+def ground_energy(lat, twoS=0, k=6):
+    """Lowest energy with total spin S=twoS/2 on the oracle lattice."""
+    ns = lat.ns
+    states = _basis(ns, twoS)
+    H = _heisenberg(ns, lat.bonds(), states)
+    n = H.shape[0]
+    if n <= 400:
+        w, v = np.linalg.eigh(H.toarray())
+    else:
+        w, v = spla.eigsh(H, k=min(k, n - 2), which="SA", tol=1e-13)
+        order = np.argsort(w)
+        w, v = w[order], v[:, order]
+    target = 0.25 * twoS * (twoS + 2)
+    for e, vec in zip(w, v.T):
+        if abs(_s2(ns, states, vec) - target) < 1e-6:
+            return float(e)
+    raise RuntimeError("no level with the requested total spin among the lowest")

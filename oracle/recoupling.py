@@ -1,0 +1,153 @@
+"""Recoupling scalars of the environment kernels (oracle).
+
+TEST INFRASTRUCTURE ONLY -- see ``oracle/__init__.py``.
+
+The reference computes each table numerically: it contracts dense CG tensors
+in the leg pattern of the matching ``tensor::contract_dmrg_*`` routine and
+takes the ratio to the output CG tensor with ``is_proportional_to``
+(dmrg_su2.cpp:777-943).  These functions restate each contraction as an
+einsum whose index letters follow the leg bookkeeping of those routines
+(derived in the docstrings), then take the same ratio.
+
+Labels: l/l' = bond of the block, s/s' = site spin, a/a' = fused (enlarged)
+leg, r/r' = right bond, k = operator leg (2j).  Pinned by the exact
+diagonalisation of small lattices in tests/test_oracle_dmrg.py: every table
+enters the environments, so a wrong factor moves the DMRG energy.
+"""
+from __future__ import annotations
+
+import math
+from functools import lru_cache
+
+import numpy as np
+
+from .su2 import cgc, identity, is_proportional_to, singlet
+
+
+def _ratio(t, ref):
+    r = is_proportional_to(t, ref)
+    if r is None:
+        raise ValueError("recoupling: contraction is not proportional to the output CG")
+    return r
+
+
+def _tri(a, b, c):
+    return abs(a - b) <= c <= a + b and (a + b + c) % 2 == 0
+
+
+@lru_cache(maxsize=None)
+def op_transform_left(jl, ja, jlp, jap, k, s):
+    """fac_operator_transformation_left[ja][k][jap][jl][jlp][s] (dmrg_su2.cpp:862-869).
+
+    contract_dmrg_operator_transformation(flag=0) (tensor_dmrg_src.cpp:318-325):
+    out[a,q,a'] = sum C(jl,s,ja)[l,x,a] C(jl,k,jlp)[l,q,m] C(jlp,s,jap)[m,x,b].
+    """
+    if not (_tri(jl, s, ja) and _tri(jlp, s, jap) and _tri(jl, k, jlp) and _tri(ja, k, jap)):
+        return 0.0
+    t = np.einsum("lxa,lqm,mxb->aqb", cgc(jl, s, ja), cgc(jl, k, jlp), cgc(jlp, s, jap))
+    return _ratio(t, cgc(ja, k, jap))
+
+
+@lru_cache(maxsize=None)
+def op_transform_right(jr, ja, jrp, jap, k, s):
+    """fac_operator_transformation_rght[ja][k][jap][jr][jrp][s] (dmrg_su2.cpp:871-876).
+
+    flag=1 branch (tensor_dmrg_src.cpp:326-333): the uu CG is C(s,jr,ja),
+    out[a,q,a'] = sum C(s,jr,ja)[x,r,a] C(jr,k,jrp)[r,q,m] C(s,jrp,jap)[x,m,b].
+    """
+    if not (_tri(s, jr, ja) and _tri(s, jrp, jap) and _tri(jr, k, jrp) and _tri(ja, k, jap)):
+        return 0.0
+    t = np.einsum("xra,rqm,xmb->aqb", cgc(s, jr, ja), cgc(jr, k, jrp), cgc(s, jrp, jap))
+    return _ratio(t, cgc(ja, k, jap))
+
+
+@lru_cache(maxsize=None)
+def op_initial_left(jl, ja, jap, k, s0, s1):
+    """fac_operator_onsite_left[ja][k][jap][jl][s0+s1*3] (dmrg_su2.cpp:817-822).
+
+    contract_dmrg_operator_initial(flag=0) (tensor_dmrg_src.cpp:85-92):
+    out[a,q,a'] = sum C(jl,s0,ja)[l,x,a] C(s0,k,s1)[x,q,y] C(jl,s1,jap)[l,y,b].
+    """
+    if not (_tri(jl, s0, ja) and _tri(jl, s1, jap) and _tri(s0, k, s1) and _tri(ja, k, jap)):
+        return 0.0
+    t = np.einsum("lxa,xqy,lyb->aqb", cgc(jl, s0, ja), cgc(s0, k, s1), cgc(jl, s1, jap))
+    return _ratio(t, cgc(ja, k, jap))
+
+
+@lru_cache(maxsize=None)
+def op_initial_right(jr, ja, jap, k, s0, s1):
+    """fac_operator_onsite_rght[ja][k][jap][jr][s0+s1*3] (dmrg_su2.cpp:825-830).
+
+    flag=1 branch (tensor_dmrg_src.cpp:93-100): contract over the site leg of
+    the first CG with the operator, cyclic shift, then over (s1, r):
+    out[a,q,a'] = sum C(s0,jr,ja)[x,r,a] C(s0,k,s1)[x,q,y] C(s1,jr,jap)[y,r,b].
+    """
+    if not (_tri(s0, jr, ja) and _tri(s1, jr, jap) and _tri(s0, k, s1) and _tri(ja, k, jap)):
+        return 0.0
+    t = np.einsum("xra,xqy,yrb->aqb", cgc(s0, jr, ja), cgc(s0, k, s1), cgc(s1, jr, jap))
+    return _ratio(t, cgc(ja, k, jap))
+
+
+@lru_cache(maxsize=None)
+def _turned_site_op(s0, k, s1):
+    """The site operator leg turned by a singlet, times sqrt(k+1).
+
+    dmrg_su2.cpp:924-926: contract(C(s0,k,s1), 1, singlet(k), 1) gives legs
+    [s1, s0, q'], shift(0,2) reorders them to [s0, q', s1].
+    """
+    t = np.einsum("xqy,pq->xpy", cgc(s0, k, s1), singlet(k))
+    return t * math.sqrt(k + 1)
+
+
+@lru_cache(maxsize=None)
+def pairup_left(ja, jl, jlp, k, s0, s1):
+    """fac_operator_pairup_left[ja][jl][jlp][k][s0+s1*3] (dmrg_su2.cpp:927-931).
+
+    contract_dmrg_operator_pairup(flag=0) (tensor_dmrg_src.cpp:180-192):
+    out[a,a'] = sum C(jl,s0,ja)[l,x,a] C(jl,k,jlp)[l,q,m] T[x,q,y] C(jlp,s1,ja)[m,y,b]
+    with T the turned site operator; ratio against identity(ja).
+    """
+    if not (_tri(jl, s0, ja) and _tri(jlp, s1, ja) and _tri(jl, k, jlp) and _tri(s0, k, s1)):
+        return 0.0
+    t = np.einsum("lxa,lqm,xqy,myb->ab", cgc(jl, s0, ja), cgc(jl, k, jlp),
+                  _turned_site_op(s0, k, s1), cgc(jlp, s1, ja))
+    return _ratio(t, identity(ja))
+
+
+@lru_cache(maxsize=None)
+def pairup_right(ja, jr, jrp, k, s0, s1):
+    """fac_operator_pairup_rght[ja][jr][jrp][k][s0+s1*3] (dmrg_su2.cpp:932-936).
+
+    flag=1 branch (tensor_dmrg_src.cpp:193-206):
+    out[a,a'] = sum C(s0,jr,ja)[x,r,a] C(jr,k,jrp)[r,q,m] T[x,q,y] C(s1,jrp,ja)[y,m,b].
+    """
+    if not (_tri(s0, jr, ja) and _tri(s1, jrp, ja) and _tri(jr, k, jrp) and _tri(s0, k, s1)):
+        return 0.0
+    t = np.einsum("xra,rqm,xqy,ymb->ab", cgc(s0, jr, ja), cgc(jr, k, jrp),
+                  _turned_site_op(s0, k, s1), cgc(s1, jrp, ja))
+    return _ratio(t, identity(ja))
+
+
+@lru_cache(maxsize=None)
+def left_to_right_psi(jl, s, jr):
+    """Factor that re-couples a singlet wavefunction across one site.
+
+    psi[(l),(s r)F=l] with the right leg built as C(s,jr,F) becomes
+    psi'[(l s)E=r,(r)] with the left leg built as C(jl,s,E): the dense
+    tensors are equal when psi'_red = f * psi_red (both sides carry the unit
+    singlet between the two legs).  Not a reference table: the reference
+    re-labels the MPS tensor without this factor when it forms the Lanczos
+    start vector (tensor_su2.cpp:749-782), so only the start vector differs.
+    """
+    src = np.einsum("lf,xrf->lxr", singlet(jl), cgc(s, jr, jl))
+    dst = np.einsum("lxe,er->lxr", cgc(jl, s, jr), singlet(jr))
+    return float(np.sum(src * dst) / np.sum(dst * dst))
+
+
+@lru_cache(maxsize=None)
+def right_to_left_psi(jl, s, jr):
+    """Inverse direction of ``left_to_right_psi``: psi[(l s)E=r,(r)] ->
+    psi'[(l),(s r)F=l]."""
+    src = np.einsum("lxe,er->lxr", cgc(jl, s, jr), singlet(jr))
+    dst = np.einsum("lf,xrf->lxr", singlet(jl), cgc(s, jr, jl))
+    return float(np.sum(src * dst) / np.sum(dst * dst))

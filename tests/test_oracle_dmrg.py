@@ -1,0 +1,111 @@
+"""Pins of the oracle DMRG (lattice tables, environments, Lanczos, SVD) against
+exact diagonalisation of the same Hamiltonian (CPU only)."""
+import numpy as np
+import pytest
+
+from oracle import dmrg as od
+from oracle import ed, heff, recoupling as rc, su2
+
+
+def test_lattice_tables_j1j2_cylinder():
+    lat = od.Lattice(4, 4, jcoup=2.0, qcoup=0.0, bdry=2)
+    # count bonds geometrically: x-bonds (lx-1)*ly, y-bonds lx*ly (periodic y),
+    # diagonals 2*(lx-1)*ly
+    nn = sum(1 for i, j, J in lat.bonds() if lat.hmap[i, j] in (1, 3))
+    dg = sum(1 for i, j, J in lat.bonds() if lat.hmap[i, j] == 2)
+    assert nn == 3 * 4 + 4 * 4
+    assert dg == 2 * 3 * 4
+    # edge columns' y-bonds carry coup[3]; couplings for jcoup=2, qcoup=0
+    assert lat.coup[1:] == [2.0, 1.0, 2.0]
+    edge = [(i, j) for i, j, _ in lat.bonds() if lat.hmap[i, j] == 3]
+    assert all(i // 4 in (0, 3) and j // 4 == i // 4 for i, j in edge) and len(edge) == 8
+    # snake order: site 4 (x=1) is y=3, so it neighbours site 3 (x=0,y=3)
+    assert lat.hmap[3, 4] == 1
+    # fll: left sites with a bond crossing the cut after site i
+    for i in range(lat.ns):
+        for j in range(i):
+            assert lat.fll[i, j] == int(np.any(lat.hmap[j, i + 1:]))
+
+
+def test_recoupling_tables_internal_consistency():
+    # transforming then re-transforming a wavefunction across a site is the identity
+    for jl in range(0, 9):
+        for jr in (jl - 1, jl + 1):
+            if jr < 0:
+                continue
+            f = rc.left_to_right_psi(jl, 1, jr)
+            g = rc.right_to_left_psi(jl, 1, jr)
+            assert abs(f * g - 1.0) < 1e-12
+    # a spin-1 operator carried through a site keeps the spin-1 coupling: the
+    # factor for j_l -> j_l with the bystander site spin unchanged is nonzero
+    assert abs(rc.op_transform_left(2, 1, 2, 1, 2, 1)) > 0
+    assert rc.op_transform_left(0, 1, 0, 1, 2, 1) == 0.0
+
+
+@pytest.mark.parametrize("lx,ly,jcoup", [(2, 2, 2.0), (3, 2, 2.0), (4, 2, 1.0), (3, 4, 2.0)])
+def test_dmrg_matches_exact_diagonalisation(lx, ly, jcoup):
+    lat = od.Lattice(lx, ly, jcoup=jcoup, bdry=2)
+    e_ed = ed.ground_energy(lat, 0)
+    dm = od.DMRG(lat, totspin=0, max_dcut=100)
+    dm.run(nsweep=3, stop=None)
+    assert abs(dm.energies[-1] - e_ed) < 1e-10 * abs(e_ed)
+
+
+def test_config1_cylinder_4x4_d200_energy():
+    # spin-1/2 J1-J2 (J2/J1 = 0.5) on 4 (open) x 4 (periodic), S = 0, D = 200:
+    # reference flags -jcoup 2 -qcoup 0, true E = 0.5 * E_ref
+    lat = od.Lattice(4, 4, jcoup=2.0, qcoup=0.0, bdry=2)
+    e_ed = ed.ground_energy(lat, 0)
+    dm = od.DMRG(lat, totspin=0, max_dcut=200)
+    per_sweep = dm.run(nsweep=5, stop=1e-6)
+    assert abs(per_sweep[-1] - e_ed) < 1e-10 * abs(e_ed)
+    e_site = 0.5 * per_sweep[-1] / 16
+    assert abs(e_site - (-0.5163270351893605)) < 1e-9
+
+
+def test_spin2_target_matches_ed():
+    lat = od.Lattice(3, 2, jcoup=2.0, bdry=2)
+    e_ed = ed.ground_energy(lat, 4)
+    dm = od.DMRG(lat, totspin=4, max_dcut=50)
+    dm.run(nsweep=2, stop=None)
+    assert abs(dm.energies[-1] - e_ed) < 1e-10
+
+
+def test_truncated_svd_against_numpy():
+    rng = np.random.default_rng(0)
+    psi = {1: rng.standard_normal((30, 20)), 3: rng.standard_normal((12, 40)),
+           5: rng.standard_normal((3, 2))}
+    U, V, bond, kept = od.truncated_svd(psi, 1, 0, 25)
+    allsv = sorted(np.concatenate([np.linalg.svd(b, compute_uv=False) for b in psi.values()]))
+    top = allsv[-25:]
+    got = sorted(np.concatenate(list(kept.values())))
+    np.testing.assert_allclose(got, top, rtol=1e-12)
+    assert sum(bond.dims) == 25
+    for j in U:
+        # p1=1, p2=0: U carries s, V orthonormal; U V^T is the best rank-d approx
+        np.testing.assert_allclose(V[j].T @ V[j], np.eye(V[j].shape[1]), atol=1e-12)
+        s = np.linalg.svd(psi[j], compute_uv=False)
+        d = V[j].shape[1]
+        err = np.linalg.norm(psi[j] - U[j] @ V[j].T)
+        assert abs(err - np.sqrt(np.sum(s[d:] ** 2))) < 1e-10
+
+
+def test_lanczos_against_eigh():
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal((40, 40))
+    A = A + A.T
+    apply = lambda v: {0: A @ v[0]}
+    e, vec, nmv = od.lanczos(apply, {0: rng.standard_normal(40)}, mlanc=40)
+    # the reference stops early once the Ritz vector's overlap with H v is
+    # within 1e-8 of one (lanczos_su2.cpp:181-186): eigenvalue error ~1e-9
+    assert abs(e - np.linalg.eigvalsh(A)[0]) < 1e-7
+    assert nmv < 40
+    # with mlanc = 2 the Ritz value is the 2x2 tridiagonal's lowest eigenvalue
+    v0 = {0: np.ones(40) / np.sqrt(40)}
+    e2, _, _ = od.lanczos(apply, v0, mlanc=2)
+    a0 = v0[0] @ A @ v0[0]
+    r = A @ v0[0] - a0 * v0[0]
+    b1 = np.linalg.norm(r)
+    q = r / b1
+    T = np.array([[a0, b1], [b1, q @ A @ q]])
+    assert abs(e2 - np.linalg.eigvalsh(T)[0]) < 1e-12
