@@ -63,6 +63,8 @@ def main():
                     choices=["cyl16x8_d2000", "cyl8x4_d1000"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--variants", default="",
+                    help="pin the GEMM tile variants 'stage1,stage2' instead of autotuning")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -83,9 +85,14 @@ def main():
     log(f"plan built: {alg / 1e9:.1f} GFLOP algorithmic, {executed / 1e9:.1f} GFLOP executed")
 
     plan.apply(dev["psi"], dev["out"])
-    chosen, t1, t2 = plan.autotune(reps=3)
-    log(f"autotune: stage1 variant {chosen[0]} ({' '.join(f'{t:.2f}' for t in t1)} ms), "
-        f"stage2 variant {chosen[1]} ({' '.join(f'{t:.2f}' for t in t2)} ms)")
+    if args.variants:
+        chosen = tuple(int(x) for x in args.variants.split(","))
+        plan.set_variants(*chosen)
+        log(f"tile variants pinned to {chosen}")
+    else:
+        chosen, t1, t2 = plan.autotune(reps=3)
+        log(f"autotune: stage1 variant {chosen[0]} ({' '.join(f'{t:.2f}' for t in t1)} ms), "
+            f"stage2 variant {chosen[1]} ({' '.join(f'{t:.2f}' for t in t2)} ms)")
     for _ in range(args.warmup):
         plan.apply(dev["psi"], dev["out"])
     ctx.sync()
