@@ -35,6 +35,7 @@ extern "C" {
 typedef struct su2_ctx su2_ctx;
 typedef struct su2_tensor su2_tensor;
 typedef struct su2_heff su2_heff;
+typedef struct su2_dmrg su2_dmrg;
 
 enum su2_status {
   SU2_OK = 0,
@@ -120,6 +121,67 @@ int su2_heff_autotune(su2_ctx* ctx, su2_heff* plan, int reps, int* chosen, float
                       int* nvariants);
 /* Pin the tile variants of the two launches (e.g. to a tuned choice). */
 int su2_heff_set_variants(su2_ctx* ctx, su2_heff* plan, int stage1, int stage2);
+
+/* Truncated SVD of a two-leg singlet wavefunction across sectors
+ * (tensor_su2::svd(tu,p1,tv,p2,wout), tensor_su2.cpp:1619-1758, with
+ * tensor::svd per sector, tensor.cpp:1472-1700): Gram matrix on MFMA, device
+ * Jacobi eigensolver, the reference's refinement of small eigenvalues, the
+ * global shell-sorted cut to max_dcut.  U = [psi leg 0, new bond] scaled by
+ * s^p1, V = [psi leg 1, new bond] scaled by s^p2 (new tensors, destroy with
+ * su2_tensor_destroy).  kept (nullable, capacity cap) receives the kept
+ * singular values sector by sector, ascending within a sector; *nkept their
+ * number; *discarded the summed squares of the cut values. */
+int su2_svd_truncate(su2_ctx* ctx, const su2_tensor* psi, double p1, double p2, int max_dcut,
+                     su2_tensor** U, su2_tensor** V, double* kept, int cap, int* nkept,
+                     double* discarded);
+
+/* Recoupling scalars of the environment kernels for one label tuple (doubled
+ * spins), the entries of the reference's fac_* tables (dmrg_su2.cpp:759-943):
+ *   kind 0 fac_operator_transformation_left  labels (jl, ja, jl', ja', k, s)
+ *   kind 1 fac_operator_transformation_rght  labels (jr, ja, jr', ja', k, s)
+ *   kind 2 fac_operator_onsite_left          labels (jl, ja, ja', k, s0, s1)
+ *   kind 3 fac_operator_onsite_rght          labels (jr, ja, ja', k, s0, s1)
+ *   kind 4 fac_operator_pairup_left          labels (ja, jl, jl', k, s0, s1)
+ *   kind 5 fac_operator_pairup_rght          labels (ja, jr, jr', k, s0, s1)
+ *   kind 6/7 wavefunction re-coupling across one site (jl, s, jr), left->right
+ *          and right->left (no reference table: used for start vectors)
+ *   kind 8 fac_hamilt_vec                    labels (j, k, j') */
+int su2_recoupling(int kind, const int* labels, double* out);
+
+/* Two-site SU(2) DMRG on an lx x ly cylinder (open x, periodic y) for total
+ * spin 2S = totspin, keeping max_dcut multiplets (dmrg_su2::dmrg_su2,
+ * dmrg_su2.cpp:31-310; flags -lx -ly -jcoup -qcoup -boundary -sec -d of
+ * main_omp.cpp:42-132).  Only qcoup = 0 is supported (no plaquette terms);
+ * boundary codes 0, 1, 2.  All environments, MPS projectors and Lanczos
+ * vectors stay in device memory. */
+int su2_dmrg_create(su2_ctx* ctx, int lx, int ly, double jcoup, double qcoup, int bdry,
+                    int totspin, int max_dcut, su2_dmrg** out);
+int su2_dmrg_destroy(su2_ctx* ctx, su2_dmrg* d);
+/* Warm-up growth (dmrg_su2::do_idmrg, dmrg_su2_omp.cpp:20-95); *energy is the
+ * last Ritz value (the reference's gs_enr + 1000, dmrg_su2.cpp:536-538). */
+int su2_dmrg_warmup(su2_ctx* ctx, su2_dmrg* d, double* energy);
+/* One sweep (dmrg_su2::sweep, dmrg_su2_omp.cpp:98-157); *energy is the Ritz
+ * value of its last (centre) step. */
+int su2_dmrg_sweep(su2_ctx* ctx, su2_dmrg* d, double* energy);
+/* Change the kept multiplet count for later steps (the reference restarts with
+ * -d/-dr to ramp D, dmrg_su2_omp.cpp:974-981). */
+int su2_dmrg_set_max_dcut(su2_ctx* ctx, su2_dmrg* d, int max_dcut);
+/* Pin the H_eff tile variants used in every step (see su2_heff_set_variants). */
+int su2_dmrg_set_heff_variants(su2_ctx* ctx, su2_dmrg* d, int stage1, int stage2);
+/* Ritz values of every step so far (the reference prints them per step,
+ * dmrg_su2.cpp:539); *n receives the count. */
+int su2_dmrg_energies(const su2_dmrg* d, double* out, int cap, int* n);
+/* Per-step record, in step order: kept multiplets of the new bond (dout of
+ * tensor_su2::svd, tensor_su2.cpp:1619-1752), the summed squared singular
+ * values its global cut drops (tensor_su2.cpp:1686-1694; the reference keeps
+ * only the kept values, wout) and the H_eff applies of the Lanczos run
+ * (lanczos_su2.cpp:237-338).  Any output may be NULL; *n receives the count. */
+int su2_dmrg_history(const su2_dmrg* d, int* kept, double* discarded, int* matvecs, int cap,
+                     int* n);
+/* Timing counters: out[0..9] = environment s, Lanczos s, SVD s, projection s,
+ * start-vector s, device matvec s, algorithmic H_eff FLOPs, matvecs, steps,
+ * last centre bond dimension. */
+int su2_dmrg_stats(const su2_dmrg* d, double* out, int cap);
 
 #ifdef __cplusplus
 }

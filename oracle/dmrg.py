@@ -245,6 +245,21 @@ def project(P, H, S):
 # --------------------------------------------------------------------------
 # truncated SVD (tensor.cpp:1472-1700, tensor_su2.cpp:1619-1758)
 # --------------------------------------------------------------------------
+def fix_signs(v):
+    """Make the first entry with |v_i| >= max|v|/2 of every column positive.
+    dsyev_ leaves eigenvector signs to the LAPACK build; this convention
+    (robust to the equal-magnitude entries symmetric lattices produce) makes
+    the all-ones warm-up start vector (tensor_su2.cpp:2367) and hence the
+    whole trajectory reproducible across eigensolvers."""
+    if v.size == 0:
+        return v
+    a = np.abs(v)
+    idx = np.argmax(a >= 0.5 * a.max(axis=0), axis=0)
+    sg = np.sign(v[idx, np.arange(v.shape[1])])
+    sg[sg == 0] = 1.0
+    return v * sg
+
+
 def _gram_eig(G):
     """Gram eigenpairs with the reference's refinement of small eigenvalues.
 
@@ -278,7 +293,7 @@ def _gram_eig(G):
             npp = j
             if npp <= 1:
                 break
-    return w, v
+    return w, fix_signs(v)
 
 
 def sector_svd(A, p1, p2, max_dcut, tol=1e-100):
@@ -325,11 +340,12 @@ def sector_svd(A, p1, p2, max_dcut, tol=1e-100):
     return u[:, m - dc:], v[:, m - dc:], s[m - dc:]
 
 
-def truncated_svd(psi, p1, p2, max_dcut):
+def truncated_svd(psi, p1, p2, max_dcut, info=None):
     """tensor_su2::svd across sectors: global shell-sorted cut to max_dcut.
 
     Returns (U {j: [D_E x d_j]}, V {j: [D_F x d_j]}, new bond, kept values per
-    sector ascending, all discarded values).
+    sector ascending).  info (a dict, optional) receives "discarded", the sum
+    of the squared singular values that the cut drops.
     """
     secs = sorted(psi)
     per = {}
@@ -345,6 +361,8 @@ def truncated_svd(psi, p1, p2, max_dcut):
     keep = {j: 0 for j in secs}
     for t in range(n - 1, k0 - 1, -1):
         keep[secs[i_sorted[t]]] += 1
+    if info is not None:
+        info["discarded"] = float(np.sum(np.square(w_sorted[:k0])))
     U, V, kept = {}, {}, {}
     tw, td = [], []
     for j in secs:
@@ -380,6 +398,8 @@ def lanczos(apply, v0, mlanc=MLANC, tol=1e-6):
         d = np.array(aal[:m])
         e = np.array(nnl[1:m])
         w, z = eigh_tridiagonal(d, e) if m > 1 else (d.copy(), np.ones((1, 1)))
+        if z[0, 0] < 0:          # Ritz vector with a positive overlap on the start vector
+            z = -z
         vec = heff.scal(z[0, 0], ff[0])
         for j in range(1, m):
             vec = heff.axpy(z[j, 0], ff[j], vec)
@@ -397,7 +417,9 @@ def lanczos(apply, v0, mlanc=MLANC, tol=1e-6):
     aal[0] = heff.inner(ff[0], f1)
     f1 = heff.axpy(-aal[0], ff[0], f1)
     nnl[1] = heff.norm(f1)
-    ff.append(heff.scal(1.0 / nnl[1], f1))
+    # an exactly invariant start vector (a 1-dim H_eff) leaves nnl = 0: the
+    # unnormalised residual is then never read again
+    ff.append(heff.scal(1.0 / nnl[1], f1) if nnl[1] > 0 else f1)
     if mlanc == 2:
         if nnl[1] < tol:
             e, vec = ritz(1)
@@ -415,7 +437,7 @@ def lanczos(apply, v0, mlanc=MLANC, tol=1e-6):
             q = heff.inner(fm, ff[j])
             fm = heff.axpy(-q, ff[j], fm)
         nnl[m] = heff.norm(fm)
-        ff.append(heff.scal(1.0 / nnl[m], fm))
+        ff.append(heff.scal(1.0 / nnl[m], fm) if nnl[m] > 0 else fm)
         if nnl[m] < tol:
             e, vec = ritz(m)
             vec, _, _ = check(vec)
@@ -464,6 +486,7 @@ class DMRG:
         self.Renv = {self.ns: (None, {})}
         self.U, self.V = {}, {}
         self.energies = []
+        self.history = []          # per step: (kept multiplets, discarded weight, matvecs)
         self.nmatvec = 0
         self.psi = None
         self.center = None
@@ -533,6 +556,13 @@ class DMRG:
         lay = {j: (El.dims[j], Er.dims[j]) for j in El.dims if j in Er.dims}
         if psi0 is None:
             psi0 = {j: np.ones(d) for j, d in lay.items()}
+        else:
+            # The vector carries every sector of the two-site space: a block the
+            # carried-over start vector lacks is null there, and the reference's
+            # operator+= (tensor_su2.cpp:157-190) fills it from H_eff's output
+            # as soon as a term reaches it (tensor_su2_dmrg_src.cpp:921-928 builds
+            # that output on the operators' full legs).
+            psi0 = {j: psi0[j] if j in psi0 else np.zeros(d) for j, d in lay.items()}
         terms = self.terms(il, ir, Sl, Sr)
         self.last_terms = (Hl, Hr, terms)
 
@@ -542,8 +572,10 @@ class DMRG:
         nele = sum(a * b for a, b in lay.values())
         e, psi, nmv = lanczos(apply, psi0, mlanc=min(MLANC, nele + 1))
         self.nmatvec += nmv
-        U, V, bond, kept = truncated_svd(psi, p1, p2, self.D)
+        info = {}
+        U, V, bond, kept = truncated_svd(psi, p1, p2, self.D, info)
         self.energies.append(e)
+        self.history.append((sum(bond.dims), info["discarded"], nmv))
         self.psi = psi
         self.center = (il, ir)
         self.kept = kept

@@ -80,7 +80,10 @@ def heff_apply(psi, hl, hr, terms):
     ``hl``/``hr`` are the block Hamiltonians (or None), ``terms`` a list of
     (left op, right op) pairs whose right (or left) side has already been
     summed with its couplings (``presum``).  Returns psi' with the same blocks
-    as ``psi``.
+    as ``psi``, which must list every sector of the two-site space (zeros where
+    the reference holds a null block): the reference's output lives on the
+    operators' full legs (tensor_su2_dmrg_src.cpp:921-928) and its operator+=
+    (tensor_su2.cpp:157-190) creates blocks the left operand lacks.
     """
     out = {j: np.zeros_like(b) for j, b in psi.items()}
     # dmrg_su2.cpp:558-570: hh[il] contracted on leg 0, hh[ir] on leg 1.

@@ -86,7 +86,8 @@ __global__ void strided_axpy_kernel(const StridedAxpy* __restrict__ desc, int nd
   const StridedAxpy d = desc[lo];
   long long i = e - d.elem_begin;
   long long r = i % d.rows, c = i / d.rows;
-  d.dst[r * d.rs + c * d.cs] += d.alpha * d.src[r + c * d.ld];
+  // a null source adds alpha (an identity block when rs/cs walk a diagonal)
+  d.dst[r * d.rs + c * d.cs] += d.src ? d.alpha * d.src[r + c * d.ld] : d.alpha;
 }
 
 unsigned grid_for(long long n) {

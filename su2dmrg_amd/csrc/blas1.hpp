@@ -21,7 +21,9 @@ void dev_axpby(Context& c, double* y, const double* x, long long n, double a, do
 double dev_dot(Context& c, const double* x, const double* y, long long n);
 
 // Batched strided scale-add: for each descriptor d,
-//   dst_d[r*rs + c*cs] += alpha_d * src_d[r + c*ld],  r < rows_d, c < cols_d.
+//   dst_d[r*rs + c*cs] += alpha_d * src_d[r + c*ld],  r < rows_d, c < cols_d
+// (src_d == nullptr adds alpha_d itself).  Descriptors of one launch must
+// write disjoint elements.
 struct StridedAxpy {
   double* dst;
   const double* src;

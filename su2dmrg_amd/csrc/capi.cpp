@@ -3,13 +3,18 @@
 
 #include <cmath>
 #include <exception>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "blas1.hpp"
+#include "cg.hpp"
 #include "common.hpp"
 #include "context.hpp"
+#include "dmrg.hpp"
 #include "heff.hpp"
+#include "recoupling.hpp"
+#include "svd.hpp"
 #include "tensor.hpp"
 
 struct su2_ctx {
@@ -17,9 +22,14 @@ struct su2_ctx {
   explicit su2_ctx(int dev) : c(dev) {}
 };
 struct su2_tensor {
-  su2::Tensor t;
+  std::unique_ptr<su2::Tensor> own;
+  su2::Tensor& t;
+  explicit su2_tensor(su2::Tensor* p) : own(p), t(*p) {}
+};
+struct su2_dmrg {
+  su2::Dmrg d;
   template <class... A>
-  explicit su2_tensor(A&&... a) : t(std::forward<A>(a)...) {}
+  explicit su2_dmrg(A&&... a) : d(std::forward<A>(a)...) {}
 };
 struct su2_heff {
   su2::HeffPlan p;
@@ -111,7 +121,7 @@ int su2_tensor_create(su2_ctx* ctx, int nleg, const int* nsec, const int* twoj, 
       pres.resize(nten);
       for (long long i = 0; i < nten; ++i) pres[i] = present[i] != 0;
     }
-    *out = new su2_tensor(&ctx->c, legs, locspin, present ? &pres : nullptr);
+    *out = new su2_tensor(new su2::Tensor(&ctx->c, legs, locspin, present ? &pres : nullptr));
   });
 }
 
@@ -374,6 +384,135 @@ int su2_heff_set_variants(su2_ctx* ctx, su2_heff* plan, int stage1, int stage2) 
     need(ctx, "ctx");
     need(plan, "plan");
     plan->p.set_variants(stage1, stage2);
+  });
+}
+
+int su2_svd_truncate(su2_ctx* ctx, const su2_tensor* psi, double p1, double p2, int max_dcut,
+                     su2_tensor** U, su2_tensor** V, double* kept, int cap, int* nkept,
+                     double* discarded) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(psi, "psi");
+    need(U, "U");
+    need(V, "V");
+    su2::SvdResult r = su2::truncated_svd(ctx->c, psi->t, p1, p2, max_dcut);
+    int n = 0;
+    for (auto& v : r.kept)
+      for (double x : v) {
+        if (kept && n < cap) kept[n] = x;
+        ++n;
+      }
+    if (nkept) *nkept = n;
+    if (discarded) *discarded = r.discarded_weight;
+    *U = new su2_tensor(r.U.release());
+    *V = new su2_tensor(r.V.release());
+  });
+}
+
+int su2_recoupling(int kind, const int* labels, double* out) {
+  return guard(nullptr, [&] {
+    need(labels, "labels");
+    need(out, "out");
+    const int* l = labels;
+    switch (kind) {
+      case 0: *out = su2::rc_op_transform_left(l[0], l[1], l[2], l[3], l[4], l[5]); break;
+      case 1: *out = su2::rc_op_transform_right(l[0], l[1], l[2], l[3], l[4], l[5]); break;
+      case 2: *out = su2::rc_op_initial_left(l[0], l[1], l[2], l[3], l[4], l[5]); break;
+      case 3: *out = su2::rc_op_initial_right(l[0], l[1], l[2], l[3], l[4], l[5]); break;
+      case 4: *out = su2::rc_pairup_left(l[0], l[1], l[2], l[3], l[4], l[5]); break;
+      case 5: *out = su2::rc_pairup_right(l[0], l[1], l[2], l[3], l[4], l[5]); break;
+      case 6: *out = su2::rc_psi_left_to_right(l[0], l[1], l[2]); break;
+      case 7: *out = su2::rc_psi_right_to_left(l[0], l[1], l[2]); break;
+      case 8: *out = su2::heff_recoupling(l[0], l[1], l[2]); break;
+      default: su2::fail(su2::kErrArgument, "recoupling kind %d", kind);
+    }
+  });
+}
+
+int su2_dmrg_create(su2_ctx* ctx, int lx, int ly, double jcoup, double qcoup, int bdry,
+                    int totspin, int max_dcut, su2_dmrg** out) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(out, "out");
+    ctx->c.make_current();
+    *out = new su2_dmrg(ctx->c, lx, ly, jcoup, qcoup, bdry, totspin, max_dcut);
+  });
+}
+
+int su2_dmrg_destroy(su2_ctx* ctx, su2_dmrg* d) {
+  return guard(ctx, [&] {
+    if (ctx) ctx->c.make_current();
+    delete d;
+  });
+}
+
+int su2_dmrg_warmup(su2_ctx* ctx, su2_dmrg* d, double* energy) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(d, "dmrg");
+    const double e = d->d.warmup();
+    if (energy) *energy = e;
+  });
+}
+
+int su2_dmrg_sweep(su2_ctx* ctx, su2_dmrg* d, double* energy) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(d, "dmrg");
+    const double e = d->d.sweep();
+    if (energy) *energy = e;
+  });
+}
+
+int su2_dmrg_set_max_dcut(su2_ctx* ctx, su2_dmrg* d, int max_dcut) {
+  return guard(ctx, [&] {
+    need(d, "dmrg");
+    SU2_CHECK(max_dcut >= 1, su2::kErrArgument, "max_dcut=%d", max_dcut);
+    d->d.set_max_dcut(max_dcut);
+  });
+}
+
+int su2_dmrg_set_heff_variants(su2_ctx* ctx, su2_dmrg* d, int stage1, int stage2) {
+  return guard(ctx, [&] {
+    need(d, "dmrg");
+    d->d.set_heff_variants(stage1, stage2);
+  });
+}
+
+int su2_dmrg_energies(const su2_dmrg* d, double* out, int cap, int* n) {
+  return guard(nullptr, [&] {
+    need(d, "dmrg");
+    const auto& e = d->d.step_energies();
+    for (int i = 0; i < (int)e.size() && i < cap; ++i)
+      if (out) out[i] = e[i];
+    if (n) *n = (int)e.size();
+  });
+}
+
+int su2_dmrg_history(const su2_dmrg* d, int* kept, double* discarded, int* matvecs, int cap,
+                     int* n) {
+  return guard(nullptr, [&] {
+    need(d, "dmrg");
+    const auto& h = d->d.history();
+    for (int i = 0; i < (int)h.size() && i < cap; ++i) {
+      if (kept) kept[i] = h[i].kept;
+      if (discarded) discarded[i] = h[i].discarded;
+      if (matvecs) matvecs[i] = h[i].matvecs;
+    }
+    if (n) *n = (int)h.size();
+  });
+}
+
+int su2_dmrg_stats(const su2_dmrg* d, double* out, int cap) {
+  return guard(nullptr, [&] {
+    need(d, "dmrg");
+    need(out, "out");
+    const su2::DmrgStats& s = d->d.stats();
+    const double v[10] = {s.t_env,      s.t_lanczos,         s.t_svd,
+                          s.t_project,  s.t_start,           s.t_matvec,
+                          s.heff_flops, (double)s.matvecs,   (double)s.steps,
+                          (double)d->d.last_bond_dim()};
+    for (int i = 0; i < 10 && i < cap; ++i) out[i] = v[i];
   });
 }
 

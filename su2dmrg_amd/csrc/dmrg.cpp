@@ -1,0 +1,636 @@
+#include "dmrg.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "blas1.hpp"
+#include "common.hpp"
+#include "context.hpp"
+#include "heff.hpp"
+#include "lanczos.hpp"
+#include "ops.hpp"
+#include "recoupling.hpp"
+#include "svd.hpp"
+
+namespace su2 {
+
+namespace {
+
+constexpr int kPhys = 1;                       // spin-1/2 sites (dmrg_su2.cpp:37)
+constexpr int kOp = 2;                         // spin-1 operator leg
+constexpr int kNfree = 2;                      // dmrg_su2.cpp:54
+constexpr int kMlanc = 5;                      // dmrg_su2.cpp:530
+const double kSigmaLeft = std::sqrt(3.0) / 2;  // sigma[0] (dmrg_su2.cpp:504)
+const double kSigmaRight = -std::sqrt(3.0) / 2;  // sigma[1] (dmrg_su2.cpp:505)
+
+Leg site_leg() { return make_leg({kPhys}, {1}, 1); }
+Leg op_leg() { return make_leg({kOp}, {1}, 1); }
+
+Leg with_dir(Leg l, int d) {
+  l.dir = d;
+  return l;
+}
+
+std::unique_ptr<Tensor> scalar_op(Context& c, const Leg& l) {
+  return std::unique_ptr<Tensor>(new Tensor(&c, {with_dir(l, 1), with_dir(l, -1)}, 0, nullptr));
+}
+std::unique_ptr<Tensor> vector_op(Context& c, const Leg& l) {
+  return std::unique_ptr<Tensor>(
+      new Tensor(&c, {with_dir(l, 1), op_leg(), with_dir(l, -1)}, 0, nullptr));
+}
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+Fused make_fused(const Leg& block, bool left) {
+  Fused F;
+  F.block = block;
+  F.fused = left ? fuse_legs(block, site_leg(), -1) : fuse_legs(site_leg(), block, -1);
+  std::map<int, long long> pos;
+  for (int jf : F.fused.twoj) pos[jf] = 0;
+  for (int s = 0; s < block.nsec(); ++s) {
+    const int jb = block.twoj[s];
+    for (int jf = std::abs(jb - kPhys); jf <= jb + kPhys; jf += 2) {
+      auto it = pos.find(jf);
+      if (it == pos.end()) continue;
+      F.off[{jb, jf}] = it->second;
+      it->second += block.dim[s];
+    }
+  }
+  return F;
+}
+
+}  // namespace
+
+std::vector<std::tuple<int, long long, int>> Fused::parts(int jf) const {
+  std::vector<std::tuple<int, long long, int>> out;
+  for (auto& kv : off)
+    if (kv.first.second == jf) out.emplace_back(kv.first.first, kv.second, block.dim_of(kv.first.first));
+  std::sort(out.begin(), out.end(),
+            [](const auto& a, const auto& b) { return std::get<1>(a) < std::get<1>(b); });
+  return out;
+}
+
+Dmrg::Dmrg(Context& c, int lx, int ly, double jcoup, double qcoup, int bdry, int totspin,
+           int max_dcut)
+    : c_(c), lat_(lx, ly, jcoup, qcoup, bdry), S_(totspin), D_(max_dcut) {
+  SU2_CHECK(totspin >= 0 && totspin <= 24, kErrArgument, "totspin 2S=%d", totspin);
+  SU2_CHECK(max_dcut >= 1, kErrArgument, "max_dcut=%d", max_dcut);
+  lb_[-1] = make_leg({0}, {1}, 1);
+  rb_[lat_.ns] = make_leg({totspin}, {1}, 1);
+  L_[-1].reset(new Env());
+  L_[-1]->bond = lb_[-1];
+  R_[lat_.ns].reset(new Env());
+  R_[lat_.ns]->bond = rb_[lat_.ns];
+}
+
+Fused Dmrg::fused_left(int i) const { return make_fused(lb_.at(i - 1), true); }
+Fused Dmrg::fused_right(int i) const { return make_fused(rb_.at(i + 1), false); }
+
+std::unique_ptr<Env> Dmrg::enlarge(const Fused& E, const Env& env, const std::vector<int>& keep,
+                                   const std::vector<std::pair<double, int>>& pair, int newsite,
+                                   bool left) {
+  Context& c = c_;
+  std::unique_ptr<Env> out(new Env());
+  out->bond = E.fused;
+  out->H = scalar_op(c, E.fused);
+  for (int k : keep) out->S[k] = vector_op(c, E.fused);
+  out->S[newsite] = vector_op(c, E.fused);
+  ScatterBatch sb;
+  Tensor& Hh = *out->H;
+  // H (x) 1: overlap_transformation, factor 1 (tensor_su2_dmrg_src.cpp:531-688)
+  if (env.H) {
+    for (int jf : E.fused.twoj) {
+      const int bh = Hh.find2(jf, jf);
+      const long long Df = E.dim(jf);
+      for (auto& p : E.parts(jf)) {
+        const int jb = std::get<0>(p);
+        const long long o = std::get<1>(p);
+        const int d = std::get<2>(p);
+        const int bs = env.H->find2(jb, jb);
+        if (bs < 0 || bh < 0) continue;
+        StridedAxpy a{};
+        a.dst = Hh.data() + Hh.blocks()[bh].offset + o + o * Df;
+        a.src = env.H->data() + env.H->blocks()[bs].offset;
+        a.rs = 1;
+        a.cs = Df;
+        a.ld = d;
+        a.rows = d;
+        a.cols = d;
+        a.alpha = 1.0;
+        sb.add(a.dst, a);
+      }
+    }
+  }
+  // sum_k J S_k . s : operator_pairup (tensor_su2_dmrg_src.cpp:691-890), with
+  // sigma[1] on the left and sigma[0] on the right (dmrg_su2_omp.cpp:645,757)
+  const double sig_pair = left ? kSigmaRight : kSigmaLeft;
+  for (auto& pr : pair) {
+    auto it = env.S.find(pr.second);
+    SU2_CHECK(it != env.S.end(), kErrStructure, "pair operator of site %d missing", pr.second);
+    const Tensor& Sk = *it->second;
+    for (int jf : E.fused.twoj) {
+      const int bh = Hh.find2(jf, jf);
+      if (bh < 0) continue;
+      const long long Df = E.dim(jf);
+      auto parts = E.parts(jf);
+      for (auto& p : parts)
+        for (auto& q : parts) {
+          const int jb = std::get<0>(p), jbp = std::get<0>(q);
+          const double f = left ? rc_pairup_left(jf, jb, jbp, kOp, kPhys, kPhys)
+                                : rc_pairup_right(jf, jb, jbp, kOp, kPhys, kPhys);
+          if (std::fabs(f) < 1e-8) continue;
+          const int bs = Sk.find3(jb, kOp, jbp);
+          if (bs < 0) continue;
+          StridedAxpy a{};
+          a.dst = Hh.data() + Hh.blocks()[bh].offset + std::get<1>(p) + std::get<1>(q) * Df;
+          a.src = Sk.data() + Sk.blocks()[bs].offset;
+          a.rs = 1;
+          a.cs = Df;
+          a.ld = std::get<2>(p);
+          a.rows = std::get<2>(p);
+          a.cols = std::get<2>(q);
+          a.alpha = pr.first * f * sig_pair;
+          sb.add(a.dst, a);
+        }
+    }
+  }
+  // S_k (x) 1: operator_transformation (tensor_su2_dmrg_src.cpp:221-402)
+  for (int k : keep) {
+    auto it = env.S.find(k);
+    SU2_CHECK(it != env.S.end(), kErrStructure, "operator of site %d missing", k);
+    const Tensor& Sk = *it->second;
+    Tensor& T = *out->S[k];
+    for (auto& blk : T.blocks()) {
+      const int jf = blk.twoj[0], jfp = blk.twoj[2];
+      const long long Df = E.dim(jf);
+      for (auto& p : E.parts(jf))
+        for (auto& q : E.parts(jfp)) {
+          const int jb = std::get<0>(p), jbp = std::get<0>(q);
+          const double f = left ? rc_op_transform_left(jb, jf, jbp, jfp, kOp, kPhys)
+                                : rc_op_transform_right(jb, jf, jbp, jfp, kOp, kPhys);
+          if (std::fabs(f) < 1e-8) continue;
+          const int bs = Sk.find3(jb, kOp, jbp);
+          if (bs < 0) continue;
+          StridedAxpy a{};
+          a.dst = T.data() + blk.offset + std::get<1>(p) + std::get<1>(q) * Df;
+          a.src = Sk.data() + Sk.blocks()[bs].offset;
+          a.rs = 1;
+          a.cs = Df;
+          a.ld = std::get<2>(p);
+          a.rows = std::get<2>(p);
+          a.cols = std::get<2>(q);
+          a.alpha = f;
+          sb.add(a.dst, a);
+        }
+    }
+  }
+  // 1 (x) s: operator_initial with sigma[0] left / sigma[1] right (tensor_su2_dmrg_src.cpp:27-218)
+  {
+    const double sig = left ? kSigmaLeft : kSigmaRight;
+    Tensor& T = *out->S[newsite];
+    for (auto& blk : T.blocks()) {
+      const int jf = blk.twoj[0], jfp = blk.twoj[2];
+      const long long Df = E.dim(jf);
+      for (auto& p : E.parts(jf))
+        for (auto& q : E.parts(jfp)) {
+          if (std::get<0>(p) != std::get<0>(q)) continue;
+          const int jb = std::get<0>(p);
+          const double f = left ? rc_op_initial_left(jb, jf, jfp, kOp, kPhys, kPhys)
+                                : rc_op_initial_right(jb, jf, jfp, kOp, kPhys, kPhys);
+          if (std::fabs(f) < 1e-8) continue;
+          StridedAxpy a{};
+          a.dst = T.data() + blk.offset + std::get<1>(p) + std::get<1>(q) * Df;
+          a.src = nullptr;     // identity on the block sector
+          a.rs = Df + 1;
+          a.cs = 0;
+          a.ld = 0;
+          a.rows = std::get<2>(p);
+          a.cols = 1;
+          a.alpha = f * sig;
+          sb.add(a.dst, a);
+        }
+    }
+  }
+  sb.run(c);
+  return out;
+}
+
+std::unique_ptr<Env> Dmrg::enlarged_left(int i, const Fused& E) {
+  const Env& env = *L_.at(i - 1);
+  std::vector<int> keep;
+  std::vector<std::pair<double, int>> pair;
+  for (int k = 0; k < i; ++k) {
+    if (lat_.keep_left(i, k) && env.S.count(k)) keep.push_back(k);
+    if (lat_.h(i, k)) pair.push_back({lat_.J(i, k), k});
+  }
+  return enlarge(E, env, keep, pair, i, true);
+}
+
+std::unique_ptr<Env> Dmrg::enlarged_right(int i, const Fused& E) {
+  const Env& env = *R_.at(i + 1);
+  std::vector<int> keep;
+  std::vector<std::pair<double, int>> pair;
+  for (int k = i + 1; k < lat_.ns; ++k) {
+    if (lat_.keep_right(i, k) && env.S.count(k)) keep.push_back(k);
+    if (lat_.h(i, k)) pair.push_back({lat_.J(i, k), k});
+  }
+  return enlarge(E, env, keep, pair, i, false);
+}
+
+std::unique_ptr<Env> Dmrg::project(const Tensor& P, const Env& hat) {
+  Context& c = c_;
+  const double t0 = now();
+  std::unique_ptr<Env> out(new Env());
+  out->bond = P.leg(1);
+  std::vector<std::pair<const Tensor*, Tensor*>> ops;
+  if (hat.H) {
+    out->H = scalar_op(c, out->bond);
+    ops.push_back({hat.H.get(), out->H.get()});
+  }
+  for (auto& kv : hat.S) {
+    out->S[kv.first] = vector_op(c, out->bond);
+    ops.push_back({kv.second.get(), out->S[kv.first].get()});
+  }
+  // Y = O[a->b] P[b], then O'[a->b] = P[a]^T Y
+  struct Job {
+    const double* O;
+    const double* Pa;
+    const double* Pb;
+    double* dst;
+    int Da, Db, da, db;
+    long long y;
+  };
+  std::vector<Job> jobs;
+  long long ytot = 0;
+  for (auto& op : ops) {
+    const Tensor& O = *op.first;
+    Tensor& T = *op.second;
+    const int last = T.nleg() - 1;
+    for (auto& tb : T.blocks()) {
+      const int a = tb.twoj[0], b = tb.twoj[last];
+      const int ob = last == 1 ? O.find2(a, b) : O.find3(a, kOp, b);
+      const int pa = P.find2(a, a), pbk = P.find2(b, b);
+      SU2_CHECK(ob >= 0 && pa >= 0 && pbk >= 0, kErrStructure, "project: missing block");
+      Job j;
+      j.O = O.data() + O.blocks()[ob].offset;
+      j.Pa = P.data() + P.blocks()[pa].offset;
+      j.Pb = P.data() + P.blocks()[pbk].offset;
+      j.dst = T.data() + tb.offset;
+      j.Da = P.blocks()[pa].dim[0];
+      j.Db = P.blocks()[pbk].dim[0];
+      j.da = P.blocks()[pa].dim[1];
+      j.db = P.blocks()[pbk].dim[1];
+      j.y = ytot;
+      ytot += (long long)j.Da * j.db;
+      jobs.push_back(j);
+    }
+  }
+  double* Y = nullptr;
+  if (ytot > 0) SU2_HIP(hipMalloc(&Y, ytot * sizeof(double)));
+  {
+    GemmBatch g;
+    for (auto& j : jobs) g.add(Y + j.y, j.Da, j.Da, j.db, j.O, j.Da, 0, j.Pb, j.Db, 0, j.Db);
+    g.run(c);
+  }
+  {
+    GemmBatch g;
+    for (auto& j : jobs) g.add(j.dst, j.da, j.da, j.db, j.Pa, j.Da, 1, Y + j.y, j.Da, 0, j.Da);
+    g.run(c);
+  }
+  SU2_HIP(hipStreamSynchronize(c.stream()));
+  if (Y) SU2_HIP(hipFree(Y));
+  stats_.t_project += now() - t0;
+  return out;
+}
+
+Dmrg::StepOut Dmrg::step(int il, int ir, std::unique_ptr<Tensor> psi0, double p1, double p2) {
+  Context& c = c_;
+  StepOut o;
+  double t0 = now();
+  o.FL = fused_left(il);
+  o.FR = fused_right(ir);
+  o.El = enlarged_left(il, o.FL);
+  o.Er = enlarged_right(ir, o.FR);
+  stats_.t_env += now() - t0;
+
+  t0 = now();
+  std::unique_ptr<Tensor> psi(
+      new Tensor(&c, {with_dir(o.FL.fused, 1), with_dir(o.FR.fused, 1)}, 0, nullptr));
+  SU2_CHECK(psi->size() > 0, kErrStructure, "step %d,%d: no common sector for spin %d", il, ir,
+            S_);
+  if (psi0) {
+    SU2_CHECK(psi0->size() == psi->size(), kErrStructure, "start vector layout mismatch");
+    dev_copy(c, psi->data(), psi0->data(), psi->size());
+  } else {
+    dev_fill(c, psi->data(), psi->size(), 1.0);   // initialize_input_vector (tensor_su2.cpp:2354-2372)
+  }
+  // H_eff terms (dmrg_su2.cpp:571-610): couplings summed once per solve
+  std::vector<HeffTerm> terms;
+  const Env& El = *o.El;
+  const Env& Er = *o.Er;
+  if (il < lat_.ns / 2) {
+    for (int j = 0; j <= il; ++j) {
+      auto lj = El.S.find(j);
+      if (lj == El.S.end()) continue;
+      HeffTerm t;
+      for (int k = ir; k < lat_.ns; ++k)
+        if (lat_.h(j, k) && Er.S.count(k)) t.right.push_back({lat_.J(j, k), Er.S.at(k).get()});
+      if (t.right.empty()) continue;
+      t.left.push_back({1.0, lj->second.get()});
+      terms.push_back(t);
+    }
+  } else {
+    for (int k = ir; k < lat_.ns; ++k) {
+      auto rk = Er.S.find(k);
+      if (rk == Er.S.end()) continue;
+      HeffTerm t;
+      for (int j = 0; j <= il; ++j)
+        if (lat_.h(j, k) && El.S.count(j)) t.left.push_back({lat_.J(j, k), El.S.at(j).get()});
+      if (t.left.empty()) continue;
+      t.right.push_back({1.0, rk->second.get()});
+      terms.push_back(t);
+    }
+  }
+  HeffPlan plan(c, *psi, El.H.get(), Er.H.get(), terms);
+  if (var1_ >= 0) plan.set_variants(var1_, var2_);
+  const int mlanc = (int)std::min<long long>(kMlanc, psi->size() + 1);
+  hipEvent_t e0, e1;
+  SU2_HIP(hipEventCreate(&e0));
+  SU2_HIP(hipEventCreate(&e1));
+  float mv_ms = 0;
+  auto apply = [&](const double* x, double* y) {
+    SU2_HIP(hipEventRecord(e0, c.stream()));
+    plan.apply_raw(x, y);
+    SU2_HIP(hipEventRecord(e1, c.stream()));
+    SU2_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    SU2_HIP(hipEventElapsedTime(&ms, e0, e1));
+    mv_ms += ms;
+  };
+  LanczosResult lr = lanczos_ground(c, apply, psi->data(), psi->size(), mlanc);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  stats_.t_matvec += mv_ms * 1e-3;
+  stats_.matvecs += lr.matvecs;
+  stats_.heff_flops += plan.algorithmic_flops() * lr.matvecs;
+  stats_.t_lanczos += now() - t0;
+
+  t0 = now();
+  SvdResult sv = truncated_svd(c, *psi, p1, p2, D_);
+  stats_.t_svd += now() - t0;
+  o.energy = lr.energy;
+  o.U = std::move(sv.U);
+  o.V = std::move(sv.V);
+  o.bond = sv.bond;
+  energies_.push_back(lr.energy);
+  history_.push_back({(int)sv.bond.total(), sv.discarded_weight, lr.matvecs});
+  last_bond_ = (int)sv.bond.total();
+  stats_.steps++;
+  return o;
+}
+
+// Lanczos start vector at (nl, nr) from the left factor A (E_nr x b) of the
+// step at (nr, nr+1): psi0 = U[nl] . A re-coupled across site nr.
+std::unique_ptr<Tensor> Dmrg::start_left(int nl, int nr, const Tensor& A, const Leg& b) {
+  Context& c = c_;
+  const double t0 = now();
+  Fused E = fused_left(nr);               // parts: lb[nl] sectors
+  Fused F = make_fused(b, false);         // F_nr = site x b
+  Fused FL = fused_left(nl);
+  const Tensor& P = *U_.at(nl);           // E_nl -> lb[nl]
+  const Leg& lbl = lb_.at(nl);
+  std::unique_ptr<Tensor> out(
+      new Tensor(&c, {with_dir(FL.fused, 1), with_dir(F.fused, 1)}, 0, nullptr));
+  ScatterBatch sb;
+  GemmBatch g;
+  std::vector<double*> tmp;
+  for (int s = 0; s < lbl.nsec(); ++s) {
+    const int jl = lbl.twoj[s];
+    const int dl = lbl.dim[s];
+    const int ob = out->find2(jl, jl), pb = P.find2(jl, jl);
+    if (ob < 0 || pb < 0 || F.dim(jl) == 0) continue;
+    const long long Df = F.dim(jl);
+    double* blk = nullptr;
+    SU2_HIP(hipMalloc(&blk, (size_t)dl * Df * sizeof(double)));
+    dev_fill(c, blk, (long long)dl * Df, 0.0);
+    tmp.push_back(blk);
+    for (auto& p : F.parts(jl)) {
+      const int jb = std::get<0>(p);
+      const int ab = A.find2(jb, jb);
+      auto it = E.off.find({jl, jb});
+      if (ab < 0 || it == E.off.end()) continue;
+      StridedAxpy a{};
+      a.dst = blk + std::get<1>(p) * dl;
+      a.src = A.data() + A.blocks()[ab].offset + it->second;
+      a.rs = 1;
+      a.cs = dl;
+      a.ld = A.blocks()[ab].dim[0];
+      a.rows = dl;
+      a.cols = std::get<2>(p);
+      a.alpha = rc_psi_right_to_left(jl, kPhys, jb);
+      sb.add(a.dst, a);
+    }
+    const int De = P.blocks()[pb].dim[0];
+    g.add(out->data() + out->blocks()[ob].offset, De, De, (int)Df, P.data() + P.blocks()[pb].offset,
+          De, 0, blk, dl, 0, dl);
+  }
+  sb.run(c);
+  g.run(c);
+  SU2_HIP(hipStreamSynchronize(c.stream()));
+  for (double* p : tmp) SU2_HIP(hipFree(p));
+  stats_.t_start += now() - t0;
+  return out;
+}
+
+// Start vector at (nl, nr) from the right factor B (F_nl x b) of the step at
+// (nl-1, nl): psi0 = B re-coupled across site nl . V[nr]^T.
+std::unique_ptr<Tensor> Dmrg::start_right(int nl, int nr, const Tensor& B, const Leg& b) {
+  Context& c = c_;
+  const double t0 = now();
+  Fused F = fused_right(nl);              // parts: rb[nr] sectors
+  Fused E = make_fused(b, true);          // E_nl = b x site
+  Fused FR = fused_right(nr);
+  const Tensor& P = *V_.at(nr);           // F_nr -> rb[nr]
+  const Leg& rbr = rb_.at(nr);
+  std::unique_ptr<Tensor> out(
+      new Tensor(&c, {with_dir(E.fused, 1), with_dir(FR.fused, 1)}, 0, nullptr));
+  ScatterBatch sb;
+  GemmBatch g;
+  std::vector<double*> tmp;
+  for (int s = 0; s < rbr.nsec(); ++s) {
+    const int jr = rbr.twoj[s];
+    const int dr = rbr.dim[s];
+    const int ob = out->find2(jr, jr), pb = P.find2(jr, jr);
+    if (ob < 0 || pb < 0 || E.dim(jr) == 0) continue;
+    const long long De = E.dim(jr);
+    double* blk = nullptr;
+    SU2_HIP(hipMalloc(&blk, (size_t)De * dr * sizeof(double)));
+    dev_fill(c, blk, De * dr, 0.0);
+    tmp.push_back(blk);
+    for (auto& p : E.parts(jr)) {
+      const int jb = std::get<0>(p);
+      const int bb = B.find2(jb, jb);
+      auto it = F.off.find({jr, jb});
+      if (bb < 0 || it == F.off.end()) continue;
+      StridedAxpy a{};
+      a.dst = blk + std::get<1>(p);
+      a.src = B.data() + B.blocks()[bb].offset + it->second;
+      a.rs = De;
+      a.cs = 1;
+      a.ld = B.blocks()[bb].dim[0];
+      a.rows = dr;
+      a.cols = std::get<2>(p);
+      a.alpha = rc_psi_left_to_right(jb, kPhys, jr);
+      sb.add(a.dst, a);
+    }
+    const int Dfr = P.blocks()[pb].dim[0];
+    g.add(out->data() + out->blocks()[ob].offset, De, (int)De, Dfr, blk, De, 0,
+          P.data() + P.blocks()[pb].offset, Dfr, 1, dr);
+  }
+  sb.run(c);
+  g.run(c);
+  SU2_HIP(hipStreamSynchronize(c.stream()));
+  for (double* p : tmp) SU2_HIP(hipFree(p));
+  stats_.t_start += now() - t0;
+  return out;
+}
+
+// Identity projector on a fused leg (the fusion isometry itself is the MPS
+// tensor while no solve has truncated the site, dmrg_su2_omp.cpp:64-68).
+static std::unique_ptr<Tensor> identity_projector(Context& c, const Leg& l) {
+  std::unique_ptr<Tensor> P(new Tensor(&c, {with_dir(l, 1), with_dir(l, -1)}, 0, nullptr));
+  ScatterBatch sb;
+  for (auto& b : P->blocks()) {
+    StridedAxpy a{};
+    a.dst = P->data() + b.offset;
+    a.src = nullptr;
+    a.rs = b.dim[0] + 1;
+    a.cs = 0;
+    a.rows = b.dim[0];
+    a.cols = 1;
+    a.alpha = 1.0;
+    sb.add(a.dst, a);
+  }
+  sb.run(c);
+  return P;
+}
+
+void Dmrg::ensure_left(int i) {
+  if (L_.count(i)) return;
+  Fused E = fused_left(i);
+  L_[i] = enlarged_left(i, E);     // untruncated: the projector is the identity
+  lb_[i] = E.fused;
+  U_[i] = identity_projector(c_, E.fused);
+}
+
+void Dmrg::ensure_right(int i) {
+  if (R_.count(i)) return;
+  Fused E = fused_right(i);
+  R_[i] = enlarged_right(i, E);
+  rb_[i] = E.fused;
+  V_[i] = identity_projector(c_, E.fused);
+}
+
+// do_idmrg (dmrg_su2_omp.cpp:20-95)
+double Dmrg::warmup() {
+  c_.make_current();
+  const int ns = lat_.ns;
+  double last = 0;
+  for (int i = 0; i <= ns / 2 - 2; ++i) {
+    ensure_left(i);
+    ensure_right(ns - 1 - i);
+    const int il = i + 1, ir = ns - 2 - i;
+    const bool final = (i == ns / 2 - 2);
+    const bool solve = ((i + 2) * 2 * kPhys >= S_ && i >= 1) || final;
+    if (!solve) continue;
+    StepOut o = step(il, ir, nullptr, final ? 1.0 : 0.0, 0.0);
+    lb_[il] = o.bond;
+    rb_[ir] = o.bond;
+    R_[ir] = project(*o.V, *o.Er);
+    V_[ir] = std::move(o.V);
+    if (final) {
+      U_[il] = std::move(o.U);
+    } else {
+      L_[il] = project(*o.U, *o.El);
+      U_[il] = std::move(o.U);
+    }
+    last = o.energy;
+  }
+  return last;
+}
+
+// dmrg_su2::sweep (dmrg_su2_omp.cpp:98-157)
+double Dmrg::sweep() {
+  c_.make_current();
+  const int ns = lat_.ns;
+  const int nf = kNfree;
+  SU2_CHECK(U_.count(ns / 2 - 1), kErrArgument, "sweep before warm-up");
+  std::unique_ptr<Tensor> A = std::move(U_[ns / 2 - 1]), B;
+  U_.erase(ns / 2 - 1);
+  Leg bond = lb_.at(ns / 2 - 1);
+  double e = energies_.empty() ? 0 : energies_.back();
+  // pass 1: centre -> left edge
+  for (int i = ns / 2; i >= nf + 2; --i) {
+    const int il = i - 2, ir = i - 1;
+    std::unique_ptr<Tensor> psi0 = start_left(il, ir, *A, bond);
+    const bool turn = (i == nf + 2);
+    StepOut o = step(il, ir, std::move(psi0), turn ? 0.0 : 1.0, turn ? 1.0 : 0.0);
+    bond = o.bond;
+    lb_[il] = bond;
+    rb_[ir] = bond;
+    if (!turn) {
+      R_[ir] = project(*o.V, *o.Er);
+      V_[ir] = std::move(o.V);
+      A = std::move(o.U);
+    } else {
+      L_[il] = project(*o.U, *o.El);
+      U_[il] = std::move(o.U);
+      B = std::move(o.V);
+    }
+    e = o.energy;
+  }
+  // pass 2: left edge -> right edge
+  for (int i = nf; i < ns - nf - 2; ++i) {
+    const int il = i + 1, ir = i + 2;
+    std::unique_ptr<Tensor> psi0 = start_right(il, ir, *B, bond);
+    const bool turn = (i == ns - nf - 3);
+    StepOut o = step(il, ir, std::move(psi0), turn ? 1.0 : 0.0, turn ? 0.0 : 1.0);
+    bond = o.bond;
+    lb_[il] = bond;
+    rb_[ir] = bond;
+    if (!turn) {
+      L_[il] = project(*o.U, *o.El);
+      U_[il] = std::move(o.U);
+      B = std::move(o.V);
+    } else {
+      R_[ir] = project(*o.V, *o.Er);
+      V_[ir] = std::move(o.V);
+      A = std::move(o.U);
+    }
+    e = o.energy;
+  }
+  // pass 3: right edge -> centre
+  int il_last = -1;
+  for (int i = ns - nf - 1; i > ns / 2; --i) {
+    const int il = i - 2, ir = i - 1;
+    std::unique_ptr<Tensor> psi0 = start_left(il, ir, *A, bond);
+    StepOut o = step(il, ir, std::move(psi0), 1.0, 0.0);
+    bond = o.bond;
+    lb_[il] = bond;
+    rb_[ir] = bond;
+    R_[ir] = project(*o.V, *o.Er);
+    V_[ir] = std::move(o.V);
+    A = std::move(o.U);
+    il_last = il;
+    e = o.energy;
+  }
+  if (il_last < 0) il_last = ns / 2 - 1;
+  U_[il_last] = std::move(A);
+  return e;
+}
+
+}  // namespace su2

@@ -1,0 +1,33 @@
+// Truncated SVD of a two-leg singlet wavefunction across its sectors
+// (tensor_su2::svd, tensor_su2.cpp:1619-1758, with tensor::svd per sector,
+// tensor.cpp:1472-1700).
+//
+// Per sector j with block A (m x n): the Gram matrix A^T A (m >= n) or A A^T
+// is formed with the grouped FP64 MFMA GEMM and diagonalised on the device;
+// eigenpairs with w/w_max < 0.1 are re-diagonalised in their projected Gram
+// until one remains (tensor.cpp:1498-1529); the other factor is A V (or
+// A^T U); singular values are sqrt(w) for w/w_max > 1e-100; kept columns are
+// scaled by s^(p1-1)/s^p2 (or s^p1/s^(p2-1)).  The largest max_dcut values
+// over all sectors survive, ordered by the reference's shell sort so ties at
+// the cut fall the same way (sort2.f:76-111).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "structure.hpp"
+#include "tensor.hpp"
+
+namespace su2 {
+
+struct SvdResult {
+  Leg bond;                              // kept sectors (2j ascending), dir -1
+  std::unique_ptr<Tensor> U;             // [psi leg 0, bond]
+  std::unique_ptr<Tensor> V;             // [psi leg 1, bond]
+  std::vector<std::vector<double>> kept; // ascending singular values per bond sector
+  double discarded_weight = 0;           // sum of squares of the cut values
+};
+
+SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int max_dcut);
+
+}  // namespace su2

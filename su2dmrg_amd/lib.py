@@ -56,6 +56,18 @@ _SIG = {
     "su2_heff_time": (_i, [_p, _p, _p, _p, _i, _dp, _dp, _dp]),
     "su2_heff_autotune": (_i, [_p, _p, _i, _ip, C.POINTER(C.c_float), _ip]),
     "su2_heff_set_variants": (_i, [_p, _p, _i, _i]),
+    "su2_svd_truncate": (_i, [_p, _p, _d, _d, _i, C.POINTER(_p), C.POINTER(_p), _dp, _i, _ip,
+                              _dp]),
+    "su2_recoupling": (_i, [_i, _ip, _dp]),
+    "su2_dmrg_create": (_i, [_p, _i, _i, _d, _d, _i, _i, _i, C.POINTER(_p)]),
+    "su2_dmrg_destroy": (_i, [_p, _p]),
+    "su2_dmrg_warmup": (_i, [_p, _p, _dp]),
+    "su2_dmrg_sweep": (_i, [_p, _p, _dp]),
+    "su2_dmrg_set_max_dcut": (_i, [_p, _p, _i]),
+    "su2_dmrg_set_heff_variants": (_i, [_p, _p, _i, _i]),
+    "su2_dmrg_energies": (_i, [_p, _dp, _i, _ip]),
+    "su2_dmrg_history": (_i, [_p, _ip, _dp, _ip, _i, _ip]),
+    "su2_dmrg_stats": (_i, [_p, _dp, _i]),
 }
 
 for _name, (_res, _args) in _SIG.items():
@@ -328,3 +340,130 @@ def upload_workload(ctx, work):
         dev_terms.append(([(1.0, tl)], [(1.0, tr)]))
     plan = HeffPlan(ctx, psi, hl, hr, dev_terms)
     return {"psi": psi, "out": out, "hl": hl, "hr": hr, "ops": keep, "plan": plan}
+
+
+# --------------------------------------------------------------------------
+# recoupling tables, truncated SVD, DMRG driver
+# --------------------------------------------------------------------------
+RC_KINDS = {"op_transform_left": 0, "op_transform_right": 1, "op_initial_left": 2,
+            "op_initial_right": 3, "pairup_left": 4, "pairup_right": 5,
+            "psi_left_to_right": 6, "psi_right_to_left": 7, "hamilt_vec": 8}
+
+
+def recoupling(kind, *labels):
+    lab = (C.c_int * 6)(*(list(labels) + [0] * (6 - len(labels))))
+    out = _d()
+    rc = lib.su2_recoupling(RC_KINDS[kind], lab, C.byref(out))
+    if rc:
+        raise Su2Error(rc, lib.su2_last_error(None).decode())
+    return out.value
+
+
+def _wrap_tensor(ctx, h):
+    t = Tensor.__new__(Tensor)
+    t.ctx = ctx
+    t.h = h
+    n = _ll()
+    nb = _i()
+    ctx.check(lib.su2_tensor_size(h, C.byref(n), C.byref(nb)))
+    t.size = n.value
+    t.blocks = []
+    for i in range(nb.value):
+        idx = _i()
+        tj = (C.c_int * 4)()
+        dm = (C.c_int * 4)()
+        off = _ll()
+        ctx.check(lib.su2_tensor_block(h, i, C.byref(idx), tj, dm, C.byref(off)))
+        t.blocks.append((idx.value, tuple(tj[:2]), tuple(dm[:2]), off.value))
+    t.legs = None
+    return t
+
+
+def svd_truncate(ctx, psi, p1, p2, max_dcut):
+    """Returns (U blocks {2j: ndarray}, V blocks, kept values, discarded weight)."""
+    hu, hv = _p(), _p()
+    cap = 1 << 16
+    kept = (C.c_double * cap)()
+    nk = _i()
+    disc = _d()
+    ctx.check(lib.su2_svd_truncate(ctx.h, psi.h, float(p1), float(p2), int(max_dcut),
+                                   C.byref(hu), C.byref(hv), kept, cap, C.byref(nk),
+                                   C.byref(disc)))
+    U = _wrap_tensor(ctx, hu)
+    V = _wrap_tensor(ctx, hv)
+    Ub = unpack_wavefunction(U, U.download())
+    Vb = unpack_wavefunction(V, V.download())
+    return Ub, Vb, list(kept[:nk.value]), disc.value
+
+
+class Dmrg:
+    """su2_dmrg: two-site SU(2) DMRG resident on the device."""
+
+    def __init__(self, ctx, lx, ly, jcoup, qcoup=0.0, bdry=2, totspin=0, max_dcut=200):
+        self.ctx = ctx
+        h = _p()
+        ctx.check(lib.su2_dmrg_create(ctx.h, lx, ly, float(jcoup), float(qcoup), bdry, totspin,
+                                      max_dcut, C.byref(h)))
+        self.h = h
+
+    def warmup(self):
+        e = _d()
+        self.ctx.check(lib.su2_dmrg_warmup(self.ctx.h, self.h, C.byref(e)))
+        return e.value
+
+    def sweep(self):
+        e = _d()
+        self.ctx.check(lib.su2_dmrg_sweep(self.ctx.h, self.h, C.byref(e)))
+        return e.value
+
+    def set_max_dcut(self, d):
+        self.ctx.check(lib.su2_dmrg_set_max_dcut(self.ctx.h, self.h, int(d)))
+
+    def set_heff_variants(self, v1, v2):
+        self.ctx.check(lib.su2_dmrg_set_heff_variants(self.ctx.h, self.h, int(v1), int(v2)))
+
+    def energies(self):
+        n = _i()
+        self.ctx.check(lib.su2_dmrg_energies(self.h, None, 0, C.byref(n)))
+        buf = (C.c_double * max(1, n.value))()
+        self.ctx.check(lib.su2_dmrg_energies(self.h, buf, n.value, C.byref(n)))
+        return list(buf[:n.value])
+
+    def history(self):
+        """Per step: (kept multiplets, discarded weight, matvecs)."""
+        n = _i()
+        self.ctx.check(lib.su2_dmrg_history(self.h, None, None, None, 0, C.byref(n)))
+        m = max(1, n.value)
+        kept, disc, mv = (_i * m)(), (C.c_double * m)(), (_i * m)()
+        self.ctx.check(lib.su2_dmrg_history(self.h, kept, disc, mv, n.value, C.byref(n)))
+        return [(kept[i], disc[i], mv[i]) for i in range(n.value)]
+
+    def stats(self):
+        out = (C.c_double * 10)()
+        self.ctx.check(lib.su2_dmrg_stats(self.h, out, 10))
+        keys = ["t_env", "t_lanczos", "t_svd", "t_project", "t_start", "t_matvec",
+                "heff_flops", "matvecs", "steps", "bond_dim"]
+        return dict(zip(keys, list(out)))
+
+    def run(self, nsweep=5, stop=1e-6):
+        """Warm-up, then sweeps until |dE| < stop (main_omp.cpp:184-202)."""
+        prev = self.warmup()
+        per = []
+        for _ in range(nsweep):
+            e = self.sweep()
+            per.append(e)
+            if stop is not None and abs(e - prev) < stop:
+                break
+            prev = e
+        return per
+
+    def close(self):
+        if self.h:
+            lib.su2_dmrg_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

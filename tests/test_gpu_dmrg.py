@@ -1,0 +1,120 @@
+"""Truncated SVD and the full two-site DMRG on the MI355X versus the oracle
+and exact diagonalisation (GPU).
+
+Step-by-step energy parity is asserted on lattices whose DMRG trajectory is
+well conditioned: perturbing every Gram matrix of the oracle by 1e-15 moves no
+energy by more than 1e-11 (pinned on the CPU in test_oracle_dmrg.py).  On the
+periodic cylinders the lattice symmetries make Gram eigenvalues degenerate;
+the warm-up's all-ones start vector (tensor_su2.cpp:2367) in that arbitrary
+basis, fed to a Lanczos run of at most five vectors, then gives
+eigensolver-dependent intermediate energies in any implementation, the
+reference's included.  There only the converged energies are compared, with
+exact diagonalisation.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import dmrg as od
+from oracle import ed
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from su2dmrg_amd import lib as L
+    c = L.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("p1,p2,dcut", [(1, 0, 60), (0, 1, 60), (1, 0, 10000), (0, 1, 25)])
+def test_truncated_svd_matches_oracle(ctx, p1, p2, dcut):
+    from su2dmrg_amd import lib as L
+    _say(f"svd p1={p1} p2={p2} dcut={dcut}")
+    rng = np.random.default_rng(4)
+    twoj = [1, 3, 5, 7]
+    da = [150, 40, 7, 1]
+    db = [90, 70, 3, 2]
+    # decaying spectra like a DMRG wavefunction
+    psi = {}
+    for j, a, b in zip(twoj, da, db):
+        u, _ = np.linalg.qr(rng.standard_normal((a, a)))
+        v, _ = np.linalg.qr(rng.standard_normal((b, b)))
+        k = min(a, b)
+        s = np.exp(-0.15 * np.arange(k)) * (1 + 0.01 * j)
+        psi[j] = (u[:, :k] * s) @ v[:, :k].T
+    t = L.wavefunction(ctx, twoj, da, db, psi)
+    U, V, kept, disc = L.svd_truncate(ctx, t, p1, p2, dcut)
+    Uo, Vo, bond, keptd = od.truncated_svd(psi, p1, p2, dcut)
+    ko = np.concatenate([keptd[j] for j in sorted(keptd)])
+    # the Gram-matrix route (the reference's) fixes a singular value s to
+    # about eps*s_max^2/s: compare absolutely against 1e-11 of the largest
+    np.testing.assert_allclose(np.array(kept), ko, rtol=0, atol=1e-11 * ko.max())
+    assert sorted(U) == sorted(Uo)
+    for j in Uo:
+        np.testing.assert_allclose(U[j] @ V[j].T, Uo[j] @ Vo[j].T, atol=1e-10)
+        # orthonormal side
+        W = V[j] if p1 == 1 else U[j]
+        np.testing.assert_allclose(W.T @ W, np.eye(W.shape[1]), atol=1e-11)
+
+
+def _run_both(ctx, lx, ly, jcoup, bdry, D, nsweep, totspin=0):
+    from su2dmrg_amd import lib as L
+    lat = od.Lattice(lx, ly, jcoup=jcoup, bdry=bdry)
+    o = od.DMRG(lat, totspin=totspin, max_dcut=D)
+    o.run(nsweep=nsweep, stop=None)
+    g = L.Dmrg(ctx, lx, ly, jcoup, 0.0, bdry, totspin, D)
+    g.run(nsweep=nsweep, stop=None)
+    return lat, o, g
+
+
+# (lx, ly, J2/J1 as jcoup, bdry, D, 2S): open lattices, the last three truncate
+STEP_CASES = [(4, 2, 1.0, 0, 100, 0), (3, 4, 2.0, 0, 20, 2), (6, 2, 2.0, 0, 10, 0),
+              (4, 4, 2.0, 0, 12, 0)]
+
+
+@pytest.mark.parametrize("lx,ly,jcoup,bdry,D,twoS", STEP_CASES)
+def test_dmrg_matches_oracle_step_by_step(ctx, lx, ly, jcoup, bdry, D, twoS):
+    _say(f"dmrg step parity {lx}x{ly} jcoup={jcoup} bdry={bdry} D={D} 2S={twoS}")
+    lat, o, g = _run_both(ctx, lx, ly, jcoup, bdry, D, 2, twoS)
+    eg = np.array(g.energies())
+    eo = np.array(o.energies)
+    assert len(eg) == len(eo)
+    np.testing.assert_allclose(eg, eo, rtol=1e-10, atol=1e-10)
+    if lx * ly <= 12 and D >= 100:
+        # untruncated: the last energy is exact
+        e_ed = ed.ground_energy(lat, twoS)
+        assert abs(eg[-1] - e_ed) < 1e-10 * abs(e_ed)
+
+
+@pytest.mark.parametrize("lx,ly,jcoup,twoS", [(4, 2, 1.0, 0), (3, 4, 2.0, 0), (3, 2, 2.0, 4)])
+def test_dmrg_cylinder_converges_to_ed(ctx, lx, ly, jcoup, twoS):
+    from su2dmrg_amd import lib as L
+    _say(f"dmrg cylinder {lx}x{ly} jcoup={jcoup} 2S={twoS} vs ED")
+    lat = od.Lattice(lx, ly, jcoup=jcoup, bdry=2)
+    e_ed = ed.ground_energy(lat, twoS)
+    g = L.Dmrg(ctx, lx, ly, jcoup, 0.0, 2, twoS, 100)
+    g.run(nsweep=2, stop=None)
+    e = g.energies()
+    assert abs(e[-1] - e_ed) < 1e-10 * abs(e_ed)
+
+
+def test_dmrg_config1_4x4_d200(ctx):
+    # J1-J2 at J2/J1 = 0.5 on 4 (open) x 4 (periodic), S = 0, D = 200, 5 sweeps
+    # with the reference's |dE| < 1e-6 stop rule: exact up to Lanczos convergence
+    from su2dmrg_amd import lib as L
+    _say("dmrg 4x4 D=200 (config 1)")
+    lat = od.Lattice(4, 4, jcoup=2.0, bdry=2)
+    e_ed = ed.ground_energy(lat, 0)
+    g = L.Dmrg(ctx, 4, 4, 2.0, 0.0, 2, 0, 200)
+    per = g.run(nsweep=5, stop=1e-6)
+    assert abs(per[-1] - e_ed) < 1e-10 * abs(e_ed)
+    assert abs(0.5 * per[-1] / 16 - (-0.5163270351893605)) < 1e-9

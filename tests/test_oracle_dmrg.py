@@ -109,3 +109,26 @@ def test_lanczos_against_eigh():
     q = r / b1
     T = np.array([[a0, b1], [b1, q @ A @ q]])
     assert abs(e2 - np.linalg.eigvalsh(T)[0]) < 1e-12
+
+
+@pytest.mark.parametrize("lx,ly,jcoup,D,twoS", [(4, 2, 1.0, 100, 0), (6, 2, 2.0, 10, 0)])
+def test_open_lattice_trajectory_is_well_conditioned(monkeypatch, lx, ly, jcoup, D, twoS):
+    # The GPU run is compared with the oracle energy by energy on open lattices.
+    # That comparison is meaningful only when the trajectory does not hinge on
+    # roundoff: perturb every Gram matrix by 1e-15 relative, symmetric noise
+    # and require every energy to move by less than 1e-11.
+    lat = od.Lattice(lx, ly, jcoup=jcoup, bdry=0)
+    ref = od.DMRG(lat, totspin=twoS, max_dcut=D)
+    ref.run(nsweep=2, stop=None)
+    eigh = np.linalg.eigh
+    rng = np.random.default_rng(11)
+
+    def noisy(G):
+        E = rng.standard_normal(G.shape)
+        return eigh(G + (E + E.T) * 1e-15 * np.abs(G).max())
+
+    monkeypatch.setattr(od.np.linalg, "eigh", noisy)
+    pert = od.DMRG(lat, totspin=twoS, max_dcut=D)
+    pert.run(nsweep=2, stop=None)
+    monkeypatch.setattr(od.np.linalg, "eigh", eigh)
+    np.testing.assert_allclose(pert.energies, ref.energies, rtol=0, atol=1e-11)
