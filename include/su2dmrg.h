@@ -168,6 +168,10 @@ int su2_dmrg_sweep(su2_ctx* ctx, su2_dmrg* d, double* energy);
 int su2_dmrg_set_max_dcut(su2_ctx* ctx, su2_dmrg* d, int max_dcut);
 /* Pin the H_eff tile variants used in every step (see su2_heff_set_variants). */
 int su2_dmrg_set_heff_variants(su2_ctx* ctx, su2_dmrg* d, int stage1, int stage2);
+/* level 1: one line per solve on stderr (step, sites, Ritz value, kept
+ * multiplets, discarded weight, matvecs, seconds); the reference logs each
+ * step to stdout (dmrg_su2_omp.cpp:113,129,145). */
+int su2_dmrg_set_verbose(su2_dmrg* d, int level);
 /* Ritz values of every step so far (the reference prints them per step,
  * dmrg_su2.cpp:539); *n receives the count. */
 int su2_dmrg_energies(const su2_dmrg* d, double* out, int cap, int* n);

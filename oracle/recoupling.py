@@ -136,8 +136,11 @@ def left_to_right_psi(jl, s, jr):
     psi'[(l s)E=r,(r)] with the left leg built as C(jl,s,E): the dense
     tensors are equal when psi'_red = f * psi_red (both sides carry the unit
     singlet between the two legs).  Not a reference table: the reference
-    re-labels the MPS tensor without this factor when it forms the Lanczos
-    start vector (tensor_su2.cpp:749-782), so only the start vector differs.
+    re-labels the MPS tensor with a fresh make_cgc and no factor when it forms
+    the Lanczos start vector (tensor_su2.cpp:749-836, then overlap_initial and
+    contract, dmrg_su2_omp.cpp:229-254).  For a spin-1/2 site (s = 1) this
+    factor is exactly 1 for every (jl, jr), so that re-labelling is exact and
+    the start vectors agree.
     """
     src = np.einsum("lf,xrf->lxr", singlet(jl), cgc(s, jr, jl))
     dst = np.einsum("lxe,er->lxr", cgc(jl, s, jr), singlet(jr))

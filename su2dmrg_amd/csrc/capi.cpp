@@ -479,6 +479,13 @@ int su2_dmrg_set_heff_variants(su2_ctx* ctx, su2_dmrg* d, int stage1, int stage2
   });
 }
 
+int su2_dmrg_set_verbose(su2_dmrg* d, int level) {
+  return guard(nullptr, [&] {
+    need(d, "dmrg");
+    d->d.set_verbose(level);
+  });
+}
+
 int su2_dmrg_energies(const su2_dmrg* d, double* out, int cap, int* n) {
   return guard(nullptr, [&] {
     need(d, "dmrg");

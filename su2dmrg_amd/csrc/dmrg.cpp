@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 
 #include "blas1.hpp"
 #include "common.hpp"
@@ -310,7 +311,8 @@ std::unique_ptr<Env> Dmrg::project(const Tensor& P, const Env& hat) {
 Dmrg::StepOut Dmrg::step(int il, int ir, std::unique_ptr<Tensor> psi0, double p1, double p2) {
   Context& c = c_;
   StepOut o;
-  double t0 = now();
+  const double t_step = now();
+  double t0 = t_step;
   o.FL = fused_left(il);
   o.FR = fused_right(ir);
   o.El = enlarged_left(il, o.FL);
@@ -389,6 +391,12 @@ Dmrg::StepOut Dmrg::step(int il, int ir, std::unique_ptr<Tensor> psi0, double p1
   energies_.push_back(lr.energy);
   history_.push_back({(int)sv.bond.total(), sv.discarded_weight, lr.matvecs});
   last_bond_ = (int)sv.bond.total();
+  if (verbose_) {
+    std::fprintf(stderr, "step %lld (%d,%d) E=%.12f kept=%d discarded=%.3e matvecs=%d %.3fs\n",
+                 (long long)energies_.size() - 1, il, ir, lr.energy, last_bond_,
+                 sv.discarded_weight, lr.matvecs, now() - t_step);
+    std::fflush(stderr);
+  }
   stats_.steps++;
   return o;
 }

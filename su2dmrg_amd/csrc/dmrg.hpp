@@ -67,6 +67,9 @@ class Dmrg {
   double warmup();
   double sweep();
   void set_max_dcut(int d) { D_ = d; }
+  // 1: one line per solve on stderr (step, sites, energy, kept multiplets,
+  // discarded weight, matvecs, seconds), as the reference logs each step
+  void set_verbose(int v) { verbose_ = v; }
   const std::vector<double>& step_energies() const { return energies_; }
   const std::vector<StepRecord>& history() const { return history_; }
   const DmrgStats& stats() const { return stats_; }
@@ -110,6 +113,7 @@ class Dmrg {
   DmrgStats stats_;
   int last_bond_ = 0;
   int var1_ = -1, var2_ = -1;
+  int verbose_ = 0;
 };
 
 }  // namespace su2

@@ -67,6 +67,7 @@ _SIG = {
     "su2_dmrg_set_heff_variants": (_i, [_p, _p, _i, _i]),
     "su2_dmrg_energies": (_i, [_p, _dp, _i, _ip]),
     "su2_dmrg_history": (_i, [_p, _ip, _dp, _ip, _i, _ip]),
+    "su2_dmrg_set_verbose": (_i, [_p, _i]),
     "su2_dmrg_stats": (_i, [_p, _dp, _i]),
 }
 
@@ -418,6 +419,9 @@ class Dmrg:
 
     def set_max_dcut(self, d):
         self.ctx.check(lib.su2_dmrg_set_max_dcut(self.ctx.h, self.h, int(d)))
+
+    def set_verbose(self, level=1):
+        self.ctx.check(lib.su2_dmrg_set_verbose(self.h, int(level)))
 
     def set_heff_variants(self, v1, v2):
         self.ctx.check(lib.su2_dmrg_set_heff_variants(self.ctx.h, self.h, int(v1), int(v2)))

@@ -132,3 +132,41 @@ def test_open_lattice_trajectory_is_well_conditioned(monkeypatch, lx, ly, jcoup,
     pert.run(nsweep=2, stop=None)
     monkeypatch.setattr(od.np.linalg, "eigh", eigh)
     np.testing.assert_allclose(pert.energies, ref.energies, rtol=0, atol=1e-11)
+
+
+def test_start_vector_recoupling_is_trivial_for_spin_half_sites():
+    # The reference carries the wavefunction across a site by re-labelling the
+    # MPS tensor's CG without any factor (tensor_su2.cpp:749-836).  That equals
+    # the exact re-coupling only if the factor is 1: check it for every spin up
+    # to the fusion cap, in both directions.
+    for jl in range(0, 39):
+        for jr in (jl - 1, jl + 1):
+            if jr < 0 or jr > 38:
+                continue
+            assert abs(rc.left_to_right_psi(jl, 1, jr) - 1.0) < 1e-12
+            assert abs(rc.right_to_left_psi(jl, 1, jr) - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("lx,ly,jcoup,bdry,twoS", [(4, 2, 1.0, 2, 0), (3, 4, 2.0, 0, 2)])
+def test_carried_start_vector_reproduces_converged_state(lx, ly, jcoup, bdry, twoS):
+    # On a converged state the wavefunction of one step, carried across a site
+    # (start_left/start_right: projector of the next site times the previous
+    # factor, with re-coupling), must be the next step's solution itself.  A
+    # wrong offset, projector or factor in any sector lowers this overlap.
+    lat = od.Lattice(lx, ly, jcoup=jcoup, bdry=bdry)
+    dm = od.DMRG(lat, totspin=twoS, max_dcut=100)
+    dm.run(nsweep=4, stop=None)
+    orig = dm.step
+    overlaps = []
+
+    def step(il, ir, psi0, p1, p2):
+        r = orig(il, ir, psi0, p1, p2)
+        if psi0 is not None:
+            a = sum(float(np.sum(psi0[j] * dm.psi[j])) for j in psi0 if j in dm.psi)
+            overlaps.append(abs(a) / heff.norm(psi0) / heff.norm(dm.psi))
+        return r
+
+    dm.step = step
+    dm.sweep()
+    assert len(overlaps) >= 4
+    assert min(overlaps) > 1 - 1e-9
