@@ -124,8 +124,10 @@ int su2_heff_set_variants(su2_ctx* ctx, su2_heff* plan, int stage1, int stage2);
 
 /* Truncated SVD of a two-leg singlet wavefunction across sectors
  * (tensor_su2::svd(tu,p1,tv,p2,wout), tensor_su2.cpp:1619-1758, with
- * tensor::svd per sector, tensor.cpp:1472-1700): Gram matrix on MFMA, device
- * Jacobi eigensolver, the reference's refinement of small eigenvalues, the
+ * tensor::svd per sector, tensor.cpp:1472-1700): the singular triplets of all
+ * sectors by one batched one-sided block Jacobi SVD on the device (the
+ * reference diagonalises Gram matrices and refines their small eigenpairs; the
+ * triplets agree to its accuracy), then the reference's keep rule and the
  * global shell-sorted cut to max_dcut.  U = [psi leg 0, new bond] scaled by
  * s^p1, V = [psi leg 1, new bond] scaled by s^p2 (new tensors, destroy with
  * su2_tensor_destroy).  kept (nullable, capacity cap) receives the kept
@@ -168,6 +170,23 @@ int su2_dmrg_sweep(su2_ctx* ctx, su2_dmrg* d, double* energy);
 int su2_dmrg_set_max_dcut(su2_ctx* ctx, su2_dmrg* d, int max_dcut);
 /* Pin the H_eff tile variants used in every step (see su2_heff_set_variants). */
 int su2_dmrg_set_heff_variants(su2_ctx* ctx, su2_dmrg* d, int stage1, int stage2);
+/* Checkpoint in the reference's file format, into directory dir
+ * (dmrg_su2::save_mps2 / save_ww / save_enr, dmrg_su2_omp.cpp:864-941,
+ * 1070-1186): uu-{ns}-{D}-{2S}-{ky}-{exci}-{i}.dat/.bin for every site (the
+ * three-leg MPS tensors with their make_cgc CG doubles), ww-...-{ns/2}.bin
+ * (max_dcut*2 doubles: the last solve's singular values, zero beyond) and
+ * enr-....dat (the last Ritz value minus 1000 appended, 16 digits).  Call
+ * after a sweep. */
+int su2_dmrg_save(su2_ctx* ctx, su2_dmrg* d, const char* dir, int ky, int exci);
+/* read_mps + prepare_sweep (dmrg_su2_omp.cpp:944-1068, 160-193): replace the
+ * state by the files of a run at file_dcut multiplets (0: this run's D; the
+ * reference's -dr restart), rebuilding every environment; su2_dmrg_sweep then
+ * continues from the centre.  *energy (nullable) receives the saved energy
+ * (read_enr adds the 1000 back, 1100-1152).  Files written by the reference
+ * load as well: a block whose CG is r times make_cgc is scaled by r.  On any
+ * error the previous state is kept. */
+int su2_dmrg_load(su2_ctx* ctx, su2_dmrg* d, const char* dir, int file_dcut, int ky, int exci,
+                  double* energy);
 /* level 1: one line per solve on stderr (step, sites, Ritz value, kept
  * multiplets, discarded weight, matvecs, seconds); the reference logs each
  * step to stdout (dmrg_su2_omp.cpp:113,129,145). */

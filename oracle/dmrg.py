@@ -345,7 +345,9 @@ def truncated_svd(psi, p1, p2, max_dcut, info=None):
 
     Returns (U {j: [D_E x d_j]}, V {j: [D_F x d_j]}, new bond, kept values per
     sector ascending).  info (a dict, optional) receives "discarded", the sum
-    of the squared singular values that the cut drops.
+    of the squared singular values that the cut drops, and "wout", the
+    reference's output array: the kept values sector by sector, then the cut
+    ones sector by sector, each ascending (tensor_su2.cpp:1707-1731).
     """
     secs = sorted(psi)
     per = {}
@@ -363,6 +365,14 @@ def truncated_svd(psi, p1, p2, max_dcut, info=None):
         keep[secs[i_sorted[t]]] += 1
     if info is not None:
         info["discarded"] = float(np.sum(np.square(w_sorted[:k0])))
+    if info is not None:
+        head, tail = [], []
+        for j in secs:
+            s_j = per[j][2]
+            cut = len(s_j) - keep[j]
+            head += list(s_j[cut:])
+            tail += list(s_j[:cut])
+        info["wout"] = np.array(head + tail)
     U, V, kept = {}, {}, {}
     tw, td = [], []
     for j in secs:
@@ -576,6 +586,7 @@ class DMRG:
         U, V, bond, kept = truncated_svd(psi, p1, p2, self.D, info)
         self.energies.append(e)
         self.history.append((sum(bond.dims), info["discarded"], nmv))
+        self.wout = info["wout"]
         self.psi = psi
         self.center = (il, ir)
         self.kept = kept

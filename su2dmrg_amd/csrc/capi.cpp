@@ -479,6 +479,26 @@ int su2_dmrg_set_heff_variants(su2_ctx* ctx, su2_dmrg* d, int stage1, int stage2
   });
 }
 
+int su2_dmrg_save(su2_ctx* ctx, su2_dmrg* d, const char* dir, int ky, int exci) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(d, "dmrg");
+    need(dir, "dir");
+    d->d.save(dir, ky, exci);
+  });
+}
+
+int su2_dmrg_load(su2_ctx* ctx, su2_dmrg* d, const char* dir, int file_dcut, int ky, int exci,
+                  double* energy) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(d, "dmrg");
+    need(dir, "dir");
+    const double e = d->d.load(dir, file_dcut, ky, exci);
+    if (energy) *energy = e;
+  });
+}
+
 int su2_dmrg_set_verbose(su2_dmrg* d, int level) {
   return guard(nullptr, [&] {
     need(d, "dmrg");

@@ -46,6 +46,8 @@ double now() {
       .count();
 }
 
+}  // namespace
+
 Fused make_fused(const Leg& block, bool left) {
   Fused F;
   F.block = block;
@@ -63,8 +65,6 @@ Fused make_fused(const Leg& block, bool left) {
   }
   return F;
 }
-
-}  // namespace
 
 std::vector<std::tuple<int, long long, int>> Fused::parts(int jf) const {
   std::vector<std::tuple<int, long long, int>> out;
@@ -390,6 +390,7 @@ Dmrg::StepOut Dmrg::step(int il, int ir, std::unique_ptr<Tensor> psi0, double p1
   o.bond = sv.bond;
   energies_.push_back(lr.energy);
   history_.push_back({(int)sv.bond.total(), sv.discarded_weight, lr.matvecs});
+  wout_ = sv.wout;
   last_bond_ = (int)sv.bond.total();
   if (verbose_) {
     std::fprintf(stderr, "step %lld (%d,%d) E=%.12f kept=%d discarded=%.3e matvecs=%d %.3fs\n",

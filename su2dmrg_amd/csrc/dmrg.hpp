@@ -19,6 +19,7 @@
 
 #include <map>
 #include <memory>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -52,6 +53,11 @@ struct StepRecord {
   int matvecs = 0;
 };
 
+// Fused leg of a block with one spin-1/2 site: block x site (left) or
+// site x block (right), with the offset of every block sector inside the
+// fused sectors (tensor_su2::fuse(b1,b2), tensor_su2.cpp:839-909).
+Fused make_fused(const Leg& block, bool left);
+
 struct DmrgStats {
   double t_env = 0, t_lanczos = 0, t_svd = 0, t_project = 0, t_start = 0;
   double heff_flops = 0;       // algorithmic FLOPs of all matvecs
@@ -66,6 +72,15 @@ class Dmrg {
        int max_dcut);
   double warmup();
   double sweep();
+  // Checkpoint in the reference's file format (dmrg_su2::save_mps2, save_ww,
+  // save_enr, dmrg_su2_omp.cpp:864-941, 1070-1186): uu-*.dat/.bin per site,
+  // ww-*.bin, enr-*.dat (appended), named after ns, max_dcut, 2S, ky, exci.
+  void save(const std::string& dir, int ky, int exci) const;
+  // read_mps + prepare_sweep (dmrg_su2_omp.cpp:944-1068, 160-193): projectors
+  // of every site from the files of a run at file_dcut multiplets (0: this
+  // run's), environments rebuilt from both edges; sweep() continues from the
+  // centre.  Returns the saved energy (read_enr, 1100-1152).
+  double load(const std::string& dir, int file_dcut, int ky, int exci);
   void set_max_dcut(int d) { D_ = d; }
   // 1: one line per solve on stderr (step, sites, energy, kept multiplets,
   // discarded weight, matvecs, seconds), as the reference logs each step
@@ -110,6 +125,7 @@ class Dmrg {
   std::map<int, std::unique_ptr<Tensor>> U_, V_;   // projectors (U at the centre may carry s)
   std::vector<double> energies_;
   std::vector<StepRecord> history_;
+  std::vector<double> wout_;                        // last solve's singular values
   DmrgStats stats_;
   int last_bond_ = 0;
   int var1_ = -1, var2_ = -1;

@@ -125,6 +125,106 @@ __device__ __forceinline__ void load_slab(const GemmSlice& s, int kt_local, int 
   }
 }
 
+// Per-thread addressing of one slice for the current tile.  Every variant has
+// NT divisible by BM, BN and BK, so element i of a thread's share of a slab
+// sits at off0 + i * stride with a uniform stride, and a whole slab moves by a
+// uniform step; only the M / N edge masks vary with i, and they are fixed for
+// the tile.  Offsets are unsigned 32-bit byte offsets from the slice's
+// (uniform) base pointer, so each load is one global_load with a scalar base
+// and a vector offset; the host checks every slice matrix spans under 4 GB.
+struct SliceAddr {
+  unsigned a0, b0;          // byte offset of element 0 at k0 = 0
+  unsigned a_safe, b_safe;  // in-matrix stand-in for elements outside the M / N edge
+  unsigned amn, bmn;        // bit i: element i lies inside the M (A) / N (B) edge
+};
+
+__device__ __forceinline__ double gload_b(const double* p, unsigned byte_off) {
+  typedef const char __attribute__((address_space(1)))* gcptr;
+  return *(gdptr)((gcptr)p + byte_off);
+}
+
+template <int BM, int BN, int BK, int WM, int WN, int PAD>
+__device__ __forceinline__ SliceAddr slice_addr(const GemmSlice& s, int m0, int n0, int M, int N) {
+  using T = Tile<BM, BN, BK, WM, WN, PAD>;
+  static_assert(T::NT % BM == 0 && T::NT % BN == 0 && T::NT % BK == 0, "affine slab layout");
+  const unsigned t = threadIdx.x;
+  const unsigned lda = (unsigned)s.lda, ldb = (unsigned)s.ldb;
+  SliceAddr ad;
+  if (!s.transA) {
+    // column-major M x K: fixed row per thread, k = t / BM + i * NT / BM
+    const int gm = m0 + (int)(t % BM);
+    ad.a0 = 8u * ((unsigned)min(gm, M - 1) + (t / BM) * lda);
+    ad.a_safe = ad.a0;
+    ad.amn = gm < M ? 0xffffffffu : 0u;
+  } else {
+    // K x M storage: fixed k per thread, row = m0 + t / BK + i * NT / BK
+    const int r0 = m0 + (int)(t / BK);
+    ad.a0 = 8u * (t % BK + (unsigned)min(r0, M - 1) * lda);
+    ad.a_safe = 8u * (t % BK + (unsigned)(M - 1) * lda);
+    ad.amn = 0;
+#pragma unroll
+    for (int i = 0; i < T::APT; ++i) ad.amn |= (unsigned)(r0 + i * (T::NT / BK) < M) << i;
+  }
+  if (!s.transB) {
+    // K x N storage: fixed k per thread, column = n0 + t / BK + i * NT / BK
+    const int c0 = n0 + (int)(t / BK);
+    ad.b0 = 8u * (t % BK + (unsigned)min(c0, N - 1) * ldb);
+    ad.b_safe = 8u * (t % BK + (unsigned)(N - 1) * ldb);
+    ad.bmn = 0;
+#pragma unroll
+    for (int i = 0; i < T::BPT; ++i) ad.bmn |= (unsigned)(c0 + i * (T::NT / BK) < N) << i;
+  } else {
+    // N x K storage: fixed column per thread, k = t / BN + i * NT / BN
+    const int gn = n0 + (int)(t % BN);
+    ad.b0 = 8u * ((unsigned)min(gn, N - 1) + (t / BN) * ldb);
+    ad.b_safe = ad.b0;
+    ad.bmn = gn < N ? 0xffffffffu : 0u;
+  }
+  return ad;
+}
+
+// Loads of a slab that lies wholly inside the slice's K range: uniform offset
+// steps, masks from the tile edges only.
+template <int BM, int BN, int BK, int WM, int WN, int PAD>
+__device__ __forceinline__ void load_slab_full(const GemmSlice& s, const SliceAddr& ad, int k0,
+                                               SlabRegs<Tile<BM, BN, BK, WM, WN, PAD>>& r) {
+  using T = Tile<BM, BN, BK, WM, WN, PAD>;
+  const unsigned lda8 = 8u * (unsigned)s.lda, ldb8 = 8u * (unsigned)s.ldb;
+  const unsigned kk = (unsigned)k0;
+  if (!s.transA) {
+    const unsigned base = ad.a0 + kk * lda8, step = (T::NT / BM) * lda8;
+#pragma unroll
+    for (int i = 0; i < T::APT; ++i) r.a[i] = gload_b(s.A, base + i * step);
+  } else {
+    const unsigned step = (T::NT / BK) * lda8;
+#pragma unroll
+    for (int i = 0; i < T::APT; ++i)
+      r.a[i] = gload_b(s.A, (((ad.amn >> i) & 1u) ? ad.a0 + i * step : ad.a_safe) + 8u * kk);
+  }
+  if (!s.transB) {
+    const unsigned step = (T::NT / BK) * ldb8;
+#pragma unroll
+    for (int i = 0; i < T::BPT; ++i)
+      r.b[i] = gload_b(s.B, (((ad.bmn >> i) & 1u) ? ad.b0 + i * step : ad.b_safe) + 8u * kk);
+  } else {
+    const unsigned base = ad.b0 + kk * ldb8, step = (T::NT / BN) * ldb8;
+#pragma unroll
+    for (int i = 0; i < T::BPT; ++i) r.b[i] = gload_b(s.B, base + i * step);
+  }
+  r.amask = ad.amn;
+  r.bmask = ad.bmn;
+}
+
+// Slab kl of a slice: the uniform-step path when the slab is full in K, the
+// clamped path for the slice's last, partial slab.
+template <int BM, int BN, int BK, int WM, int WN, int PAD>
+__device__ __forceinline__ void load_slab_any(const GemmSlice& s, const SliceAddr& ad, int kl,
+                                              int m0, int n0, int M, int N,
+                                              SlabRegs<Tile<BM, BN, BK, WM, WN, PAD>>& r) {
+  if ((kl + 1) * BK <= s.K) load_slab_full<BM, BN, BK, WM, WN, PAD>(s, ad, kl * BK, r);
+  else load_slab<BM, BN, BK, WM, WN, PAD>(s, kl, m0, n0, M, N, r);
+}
+
 template <int BM, int BN, int BK, int WM, int WN, int PAD>
 __device__ __forceinline__ void store_slab(const GemmSlice& s,
                                            const SlabRegs<Tile<BM, BN, BK, WM, WN, PAD>>& r,
@@ -160,8 +260,10 @@ __device__ __forceinline__ void store_slab(const GemmSlice& s,
   }
 }
 
+// Two waves per SIMD (at most 256 registers per lane): one wave's MFMAs run
+// while the other loads, stores its next slab or waits at the barrier.
 template <int BM, int BN, int BK, int WM, int WN, int PAD>
-__global__ __launch_bounds__(WM * WN * 64) void grouped_gemm_f64_kernel(
+__global__ __launch_bounds__(WM * WN * 64, 2) void grouped_gemm_f64_kernel(
     const GemmProblem* __restrict__ probs, int nprob, const GemmSlice* __restrict__ slices,
     double* __restrict__ ws, long long total_units) {
   using T = Tile<BM, BN, BK, WM, WN, PAD>;
@@ -176,8 +278,8 @@ __global__ __launch_bounds__(WM * WN * 64) void grouped_gemm_f64_kernel(
   const int ntile = P.tiles_m * P.tiles_n;
   const int ks = (int)(u / ntile);
   const int tile = (int)(u % ntile);
-  const int tm = tile % P.tiles_m;
-  const int tn = tile / P.tiles_m;
+  const int tm = P.n_fast ? tile / P.tiles_n : tile % P.tiles_m;
+  const int tn = P.n_fast ? tile % P.tiles_n : tile / P.tiles_m;
   const int m0 = tm * BM, n0 = tn * BN;
   const int kt_lo = (int)((long long)P.kt_total * ks / P.ksplit);
   const int kt_hi = (int)((long long)P.kt_total * (ks + 1) / P.ksplit);
@@ -199,9 +301,10 @@ __global__ __launch_bounds__(WM * WN * 64) void grouped_gemm_f64_kernel(
     while (s + 1 < P.slice_end && slices[s + 1].kt_begin <= kt_lo) ++s;
     GemmSlice S = slices[s];
     int kl = kt_lo - S.kt_begin;
+    SliceAddr ad = slice_addr<BM, BN, BK, WM, WN, PAD>(S, m0, n0, P.M, P.N);
 
     SlabRegs<T> regs;
-    load_slab<BM, BN, BK, WM, WN, PAD>(S, kl, m0, n0, P.M, P.N, regs);
+    load_slab_any<BM, BN, BK, WM, WN, PAD>(S, ad, kl, m0, n0, P.M, P.N, regs);
     store_slab<BM, BN, BK, WM, WN, PAD>(S, regs, As[0], Bs[0]);
     __syncthreads();
 
@@ -217,8 +320,9 @@ __global__ __launch_bounds__(WM * WN * 64) void grouped_gemm_f64_kernel(
           ++s;
           Sn = slices[s];
           kln = 0;
+          ad = slice_addr<BM, BN, BK, WM, WN, PAD>(Sn, m0, n0, P.M, P.N);
         }
-        load_slab<BM, BN, BK, WM, WN, PAD>(Sn, kln, m0, n0, P.M, P.N, regs);
+        load_slab_any<BM, BN, BK, WM, WN, PAD>(Sn, ad, kln, m0, n0, P.M, P.N, regs);
       }
       const double* Ab = As[buf];
       const double* Bb = Bs[buf];
@@ -352,6 +456,13 @@ void GroupedGemm::add_slice(int problem, const double* A, long long lda, int tra
   SU2_CHECK(K > 0, kErrArgument, "gemm: empty slice");
   SU2_CHECK(lda >= (transA ? K : std::max(p.M, 1)) && ldb >= (transB ? std::max(p.N, 1) : K),
             kErrArgument, "gemm: leading dimension too small");
+  // the kernel addresses a slice with unsigned 32-bit byte offsets (plus one
+  // tile of margin past the M / N edge)
+  const long long lim = (1LL << 29) - 1;
+  const long long a_span = transA ? (long long)(p.M + 256) * lda + K : (long long)(K + 256) * lda + p.M;
+  const long long b_span = transB ? (long long)(K + 256) * ldb + p.N : (long long)(p.N + 256) * ldb + K;
+  SU2_CHECK(a_span < lim && b_span < lim, kErrCapacity,
+            "gemm: slice matrix over 4 GB (M=%d N=%d K=%d)", p.M, p.N, K);
   GemmSlice sl{};
   sl.A = A;
   sl.B = B;
@@ -391,6 +502,9 @@ void GroupedGemm::set_variant(int v) {
     p.kt_total = kt;
     p.tiles_m = (p.M + V.bm - 1) / V.bm;
     p.tiles_n = (p.N + V.bn - 1) / V.bn;
+    // keep the larger operand's panel fixed across neighbouring tiles, so it
+    // streams from HBM about once while the smaller one is re-read from cache
+    p.n_fast = p.M > p.N ? 1 : 0;
     work += (double)p.tiles_m * p.tiles_n * kt;
   }
   // split K so that the launch has about target_units comparable work units

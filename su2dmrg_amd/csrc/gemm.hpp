@@ -32,6 +32,7 @@ struct GemmProblem {
   int M, N;
   int slice_begin, slice_end;
   int tiles_m, tiles_n;
+  int n_fast;        // 1: consecutive tiles walk N (A panel reused), 0: walk M
   int ksplit;
   int kt_total;      // k-tiles over all slices (each slice padded to the tile)
   long long ws_offset;  // doubles into the split-K workspace (ksplit > 1)

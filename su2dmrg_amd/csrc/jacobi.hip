@@ -1,0 +1,525 @@
+// One-sided block Jacobi SVD, see jacobi.hpp.
+#include "jacobi.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+
+#include "common.hpp"
+#include "context.hpp"
+
+namespace su2 {
+
+namespace {
+
+constexpr int CB = 32;          // columns per block
+constexpr int NP = 2 * CB;      // largest pair problem
+constexpr int RCH = 32;         // rows per staged chunk
+constexpr int NT = 256;         // threads per pair workgroup
+constexpr int INNER_SWEEPS = 40;
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+struct BlockDesc {
+  double* W;        // L x k, ld L
+  double* V;        // k x k, ld k
+  int L, k;
+  double normF;     // Frobenius norm of the block (absolute floor scale)
+};
+
+struct PairJob {
+  int blk;          // index into the block table
+  int p0, bp;       // first column block
+  int q0, bq;       // second column block (bq = 0: the first block alone)
+};
+
+__device__ __forceinline__ int rr_p(int r, int k, int m) { return k == 0 ? m - 1 : (r + k) % (m - 1); }
+__device__ __forceinline__ int rr_q(int r, int k, int m) {
+  return k == 0 ? r : (r - k + (m - 1)) % (m - 1);
+}
+
+// Orthogonalise the columns of one pair of column blocks of W: Gram matrix,
+// in-LDS cyclic Jacobi, rotation of the pair's columns of W and V.  Records
+// the pair's largest scaled off-diagonal Gram entry (before rotating) in
+// off[blk] (as ordered bits of a non-negative double).
+__global__ __launch_bounds__(NT) void jacobi_pair_kernel(const BlockDesc* __restrict__ blocks,
+                                                         const PairJob* __restrict__ jobs,
+                                                         double tol_rel, double tol_abs,
+                                                         unsigned long long* __restrict__ off) {
+  __shared__ double G[NP][NP + 1];
+  __shared__ double R[NP][NP + 1];
+  __shared__ double X[RCH][NP + 1];
+  __shared__ double cs[NP / 2], sn[NP / 2];
+  __shared__ int pp[NP / 2], qq[NP / 2];
+  __shared__ int colmap[NP];
+  __shared__ double red[NT];
+  __shared__ int nrot;
+
+  const PairJob J = jobs[blockIdx.x];
+  const BlockDesc B = blocks[J.blk];
+  const int n = J.bp + J.bq;
+  const int t = threadIdx.x;
+  if (n < 2) return;
+  for (int c = t; c < n; c += NT) colmap[c] = c < J.bp ? J.p0 + c : J.q0 + (c - J.bp);
+
+  // ---- Gram matrix G = X^T X of the pair's columns on the matrix cores ----
+  // 32-row chunks of the columns are staged in LDS (columns >= n zero); wave
+  // w accumulates the 16 x 16 tiles (w, 0..3) with v_mfma_f64_16x16x4f64:
+  // A = X^T fragment (lane l: X[r + l/16][16 w + l%16]), B = X fragment
+  // (lane l: X[r + l/16][16 j + l%16]), D lane l: G[16 w + l/16 + 4 q][16 j + l%16].
+  const int lane = t & 63, wave = t >> 6;
+  f64x4 acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] = f64x4{0.0, 0.0, 0.0, 0.0};
+  __syncthreads();
+  for (int r0 = 0; r0 < B.L; r0 += RCH) {
+    const int nr = min(RCH, B.L - r0);
+    for (int e = t; e < RCH * NP; e += NT) {
+      const int rr = e % RCH, c = e / RCH;
+      X[rr][c] = (rr < nr && c < n) ? B.W[(r0 + rr) + (long long)colmap[c] * B.L] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k4 = 0; k4 < RCH; k4 += 4) {
+      const int row = k4 + (lane >> 4);
+      const double a = X[row][16 * wave + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, X[row][16 * j + (lane & 15)], acc[j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) G[16 * wave + (lane >> 4) + 4 * q][16 * j + (lane & 15)] = acc[j][q];
+  __syncthreads();
+
+  // ---- convergence measure of this pair ----
+  double worst = 0.0;
+  bool rotate = false;
+  for (int e = t; e < n * n; e += NT) {
+    const int a = e % n, b = e / n;
+    if (a >= b) continue;
+    const double g = fabs(G[a][b]);
+    const double na = sqrt(G[a][a]), nb = sqrt(G[b][b]);
+    // columns at the rounding level of the block carry no direction (the SVD
+    // zeroes them); they do not hold the iteration open
+    if (fmin(na, nb) <= 1e-13 * B.normF) continue;
+    const double scale = na * nb;
+    // relative orthogonality, or the rounding floor of columns that carry
+    // absolute errors of order eps ||A||
+    const double floor_ab = tol_abs * B.normF * (na + nb);
+    if (g > fmax(tol_rel * scale, floor_ab)) {
+      rotate = true;
+      worst = fmax(worst, g / scale);
+    }
+  }
+  red[t] = worst + (rotate ? 4.0 : 0.0);   // +4 marks "needs a rotation" (ratios are <= 1)
+  __syncthreads();
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if (t < w) red[t] = fmax(red[t], red[t + w]);
+    __syncthreads();
+  }
+  const double top = red[0];
+  if (t == 0) {
+    const double ratio = top >= 4.0 ? top - 4.0 : top;
+    // a pair that still needs a rotation reports at least 2 (above any ratio)
+    const double rep = top >= 4.0 ? 2.0 + ratio : ratio;
+    atomicMax(off + J.blk, (unsigned long long)__double_as_longlong(rep));
+  }
+  if (top < 4.0) return;   // orthogonal to tolerance: nothing to rotate
+
+  // ---- eigenvectors of G by cyclic Jacobi (rotations relative to the diagonal) ----
+  for (int e = t; e < NP * NP; e += NT) {
+    const int a = e % NP, b = e / NP;
+    R[a][b] = a == b ? 1.0 : 0.0;
+  }
+  const int m = n + (n & 1);
+  const int npair = m / 2;
+  __syncthreads();
+  for (int sweep = 0; sweep < INNER_SWEEPS; ++sweep) {
+    if (t == 0) nrot = 0;
+    __syncthreads();
+    for (int r = 0; r < m - 1; ++r) {
+      if (t < npair) {
+        const int p = rr_p(r, t, m), q = rr_q(r, t, m);
+        double c = 1.0, s = 0.0;
+        if (p < n && q < n) {
+          const double apq = G[p][q];
+          const double app = G[p][p], aqq = G[q][q];
+          if (fabs(apq) > 1e-15 * sqrt(fabs(app * aqq)) && fabs(apq) > 1e-300) {
+            const double tau = (aqq - app) / (2.0 * apq);
+            const double tt = fabs(tau) > 1e150 ? 0.5 / tau
+                                                : (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            c = 1.0 / sqrt(1.0 + tt * tt);
+            s = tt * c;
+            atomicAdd(&nrot, 1);
+          }
+        }
+        pp[t] = p;
+        qq[t] = q;
+        cs[t] = c;
+        sn[t] = s;
+      }
+      __syncthreads();
+      for (int e = t; e < npair * n; e += NT) {   // G <- G J
+        const int k = e / n, row = e % n;
+        const int p = pp[k], q = qq[k];
+        if (p >= n || q >= n || sn[k] == 0.0) continue;
+        const double c = cs[k], s = sn[k];
+        const double x = G[row][p], y = G[row][q];
+        G[row][p] = c * x - s * y;
+        G[row][q] = s * x + c * y;
+      }
+      __syncthreads();
+      for (int e = t; e < npair * n; e += NT) {   // G <- J^T G, R <- R J
+        const int k = e / n, col = e % n;
+        const int p = pp[k], q = qq[k];
+        if (p >= n || q >= n || sn[k] == 0.0) continue;
+        const double c = cs[k], s = sn[k];
+        const double x = G[p][col], y = G[q][col];
+        G[p][col] = c * x - s * y;
+        G[q][col] = s * x + c * y;
+        const double u = R[col][p], v = R[col][q];
+        R[col][p] = c * u - s * v;
+        R[col][q] = s * u + c * v;
+      }
+      __syncthreads();
+    }
+    if (nrot == 0) break;
+    __syncthreads();
+  }
+
+  // ---- rotate the pair's columns: W[:, cols] <- W[:, cols] R, same for V ----
+  // Y = X R per 32-row chunk on the matrix cores: wave w owns the row tile
+  // w % 2 and the column tiles 2 (w / 2) and 2 (w / 2) + 1; A = X fragment
+  // (lane l: X[16 rt + l%16][k + l/16]), B = R fragment (lane l: R[k + l/16][16 ct + l%16]).
+  // The result goes back through LDS so each column segment is written whole.
+  const int rt = wave & 1, ct0 = 2 * (wave >> 1);
+  for (int which = 0; which < 2; ++which) {
+    double* M = which == 0 ? B.W : B.V;
+    const int rows = which == 0 ? B.L : B.k;
+    const long long ld = which == 0 ? B.L : B.k;
+    for (int r0 = 0; r0 < rows; r0 += RCH) {
+      const int nr = min(RCH, rows - r0);
+      __syncthreads();
+      for (int e = t; e < RCH * NP; e += NT) {
+        const int rr = e % RCH, c = e / RCH;
+        X[rr][c] = (rr < nr && c < n) ? M[(r0 + rr) + (long long)colmap[c] * ld] : 0.0;
+      }
+      __syncthreads();
+      f64x4 y[2] = {f64x4{0.0, 0.0, 0.0, 0.0}, f64x4{0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+      for (int k = 0; k < NP; k += 4) {
+        const double a = X[16 * rt + (lane & 15)][k + (lane >> 4)];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+          y[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, R[k + (lane >> 4)][16 * (ct0 + u) + (lane & 15)],
+                                                      y[u], 0, 0, 0);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          X[16 * rt + (lane >> 4) + 4 * q][16 * (ct0 + u) + (lane & 15)] = y[u][q];
+      __syncthreads();
+      for (int e = t; e < RCH * n; e += NT) {
+        const int rr = e % RCH, c = e / RCH;
+        if (rr < nr) M[(r0 + rr) + (long long)colmap[c] * ld] = X[rr][c];
+      }
+    }
+  }
+}
+
+// W = A (transpose = 0) or A^T (transpose = 1); A is m x n with ld m
+__global__ void load_work_kernel(const double* __restrict__ A, int m, int n, int transpose,
+                                 double* __restrict__ W) {
+  __shared__ double tile[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x, ty = threadIdx.y;   // 32 x 8
+  if (!transpose) {
+    for (int j = ty; j < 32; j += 8) {
+      const int r = bx + tx, c = by + j;
+      if (r < m && c < n) W[r + (long long)c * m] = A[r + (long long)c * m];
+    }
+    return;
+  }
+  // W (n x m) = A^T: read A tile (rows bx.., cols by..), write transposed
+  for (int j = ty; j < 32; j += 8) {
+    const int r = bx + tx, c = by + j;
+    tile[j][tx] = (r < m && c < n) ? A[r + (long long)c * m] : 0.0;
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int wr = by + tx, wc = bx + j;   // W row = A column, W column = A row
+    if (wr < n && wc < m) W[wr + (long long)wc * n] = tile[tx][j];
+  }
+}
+
+__global__ void eye_kernel(double* __restrict__ V, int k) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < (long long)k * k) V[e] = (e % k) == (e / k) ? 1.0 : 0.0;
+}
+
+// squared column norms of W (one workgroup per column)
+__global__ __launch_bounds__(256) void colnorm_kernel(const double* __restrict__ W, int L,
+                                                      double* __restrict__ out) {
+  __shared__ double s[256];
+  const double* col = W + (long long)blockIdx.x * L;
+  double a = 0.0;
+  for (int r = threadIdx.x; r < L; r += 256) a += col[r] * col[r];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+}
+
+// dst[:, c] = sign[c] * src[:, perm[c]]
+__global__ void permute_cols_kernel(const double* __restrict__ src, double* __restrict__ dst,
+                                    const int* __restrict__ perm, const double* __restrict__ sign,
+                                    int rows, int cols) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)rows * cols) return;
+  const int r = (int)(e % rows), c = (int)(e / rows);
+  dst[e] = sign[c] * src[r + (long long)perm[c] * rows];
+}
+
+__global__ void fill_u64_kernel(unsigned long long* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// sign of the first entry with |v| >= max|v|/2 of each column (as +-1)
+__global__ __launch_bounds__(256) void col_sign_kernel(const double* __restrict__ V, int rows,
+                                                       double* __restrict__ sign) {
+  __shared__ double sv[256];
+  __shared__ int si[256];
+  const double* col = V + (long long)blockIdx.x * rows;
+  double best = 0.0;
+  for (int r = threadIdx.x; r < rows; r += 256) best = fmax(best, fabs(col[r]));
+  sv[threadIdx.x] = best;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sv[threadIdx.x] = fmax(sv[threadIdx.x], sv[threadIdx.x + w]);
+    __syncthreads();
+  }
+  const double half = 0.5 * sv[0];
+  int first = 0x7fffffff;
+  for (int r = threadIdx.x; r < rows; r += 256)
+    if (fabs(col[r]) >= half && r < first) first = r;
+  si[threadIdx.x] = first;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) si[threadIdx.x] = min(si[threadIdx.x], si[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sign[blockIdx.x] = (si[0] < rows && col[si[0]] < 0.0) ? -1.0 : 1.0;
+}
+
+unsigned nblk(long long n, int t) { return (unsigned)std::max<long long>(1, (n + t - 1) / t); }
+
+struct DevMem {
+  void* p = nullptr;
+  explicit DevMem(size_t bytes) {
+    if (bytes) SU2_HIP(hipMalloc(&p, bytes));
+  }
+  ~DevMem() {
+    if (p) (void)hipFree(p);
+  }
+  DevMem(const DevMem&) = delete;
+  DevMem& operator=(const DevMem&) = delete;
+};
+
+}  // namespace
+
+void jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol, int max_sweeps) {
+  hipStream_t st = c.stream();
+  const int nb = (int)blocks.size();
+  if (nb == 0) return;
+  std::vector<int> L(nb), K(nb);
+  std::vector<long long> noff(nb + 1, 0);
+  for (int b = 0; b < nb; ++b) {
+    SU2_CHECK(blocks[b].m >= 1 && blocks[b].n >= 1, kErrArgument, "jacobi: empty block");
+    L[b] = std::max(blocks[b].m, blocks[b].n);
+    K[b] = std::min(blocks[b].m, blocks[b].n);
+    noff[b + 1] = noff[b] + K[b];
+  }
+  // second buffers: each sweep starts by moving the columns into descending
+  // norm order (de Rijk's ordering; it cuts the sweep count severalfold)
+  std::vector<std::unique_ptr<DevMem>> W2(nb), V2(nb);
+  std::vector<double*> Wc(nb), Vc(nb), Wo(nb), Vo(nb);
+  for (int b = 0; b < nb; ++b) {
+    JacobiBlock& J = blocks[b];
+    W2[b].reset(new DevMem(sizeof(double) * (size_t)L[b] * K[b]));
+    V2[b].reset(new DevMem(sizeof(double) * (size_t)K[b] * K[b]));
+    Wc[b] = J.W;
+    Vc[b] = J.V;
+    Wo[b] = (double*)W2[b]->p;
+    Vo[b] = (double*)V2[b]->p;
+    hipLaunchKernelGGL(load_work_kernel, dim3(nblk(J.m, 32), nblk(J.n, 32)), dim3(32, 8), 0, st,
+                       J.A, J.m, J.n, J.m < J.n ? 1 : 0, J.W);
+    hipLaunchKernelGGL(eye_kernel, dim3(nblk((long long)K[b] * K[b], 256)), dim3(256), 0, st, J.V,
+                       K[b]);
+  }
+  SU2_HIP(hipGetLastError());
+  DevMem dn(sizeof(double) * noff[nb]), dperm(sizeof(int) * noff[nb]),
+      done(sizeof(double) * noff[nb]);
+  std::vector<double> hn(noff[nb]);
+  std::vector<int> hp(noff[nb]);
+  {
+    std::vector<double> ones(noff[nb], 1.0);
+    SU2_HIP(hipMemcpyAsync(done.p, ones.data(), sizeof(double) * noff[nb], hipMemcpyHostToDevice,
+                           st));
+  }
+  auto norms = [&]() {
+    for (int b = 0; b < nb; ++b)
+      hipLaunchKernelGGL(colnorm_kernel, dim3(K[b]), dim3(256), 0, st, Wc[b], L[b],
+                         (double*)dn.p + noff[b]);
+    SU2_HIP(hipMemcpyAsync(hn.data(), dn.p, sizeof(double) * noff[nb], hipMemcpyDeviceToHost, st));
+    SU2_HIP(hipStreamSynchronize(st));
+  };
+  std::vector<BlockDesc> desc(nb);
+  norms();
+  for (int b = 0; b < nb; ++b) {
+    double g = 0;
+    for (int i = 0; i < K[b]; ++i) g += hn[noff[b] + i];
+    desc[b] = BlockDesc{nullptr, nullptr, L[b], K[b], std::sqrt(g)};
+  }
+  // round-robin schedule of column-block pairs, merged across blocks by round
+  std::vector<std::vector<PairJob>> rounds;
+  for (int b = 0; b < nb; ++b) {
+    const int ncb = (K[b] + CB - 1) / CB;
+    auto width = [&](int i) { return i >= ncb ? 0 : std::min(CB, K[b] - i * CB); };
+    if (ncb == 1) {
+      if (rounds.empty()) rounds.resize(1);
+      if (K[b] >= 2) rounds[0].push_back(PairJob{b, 0, K[b], 0, 0});
+      continue;
+    }
+    const int mp = ncb + (ncb & 1);
+    if ((int)rounds.size() < mp - 1) rounds.resize(mp - 1);
+    for (int r = 0; r < mp - 1; ++r)
+      for (int k = 0; k < mp / 2; ++k) {
+        int P = (k == 0) ? mp - 1 : (r + k) % (mp - 1);
+        int Q = (k == 0) ? r : (r - k + (mp - 1)) % (mp - 1);
+        if (width(P) == 0) std::swap(P, Q);
+        if (width(P) == 0) continue;
+        rounds[r].push_back(PairJob{b, P * CB, width(P), width(Q) ? Q * CB : 0, width(Q)});
+      }
+  }
+  std::vector<int> round_off(rounds.size() + 1, 0);
+  std::vector<PairJob> all;
+  for (size_t r = 0; r < rounds.size(); ++r) {
+    round_off[r] = (int)all.size();
+    all.insert(all.end(), rounds[r].begin(), rounds[r].end());
+  }
+  round_off[rounds.size()] = (int)all.size();
+  DevMem ddesc(sizeof(BlockDesc) * nb), djobs(sizeof(PairJob) * std::max<size_t>(1, all.size())),
+      doff(sizeof(unsigned long long) * nb);
+  if (!all.empty())
+    SU2_HIP(hipMemcpyAsync(djobs.p, all.data(), sizeof(PairJob) * all.size(),
+                           hipMemcpyHostToDevice, st));
+  std::vector<unsigned long long> hoff(nb);
+  bool converged = all.empty();
+  double worst = 0;
+  for (int sweep = 0; sweep < max_sweeps && !converged; ++sweep) {
+    // descending column norms
+    if (sweep > 0) norms();
+    for (int b = 0; b < nb; ++b) {
+      int* p = hp.data() + noff[b];
+      const double* w = hn.data() + noff[b];
+      std::iota(p, p + K[b], 0);
+      std::stable_sort(p, p + K[b], [&](int x, int y) { return w[x] > w[y]; });
+    }
+    SU2_HIP(hipMemcpyAsync(dperm.p, hp.data(), sizeof(int) * noff[nb], hipMemcpyHostToDevice, st));
+    for (int b = 0; b < nb; ++b) {
+      const int* pb = (const int*)dperm.p + noff[b];
+      const double* one = (const double*)done.p + noff[b];
+      hipLaunchKernelGGL(permute_cols_kernel, dim3(nblk((long long)L[b] * K[b], 256)), dim3(256),
+                         0, st, Wc[b], Wo[b], pb, one, L[b], K[b]);
+      hipLaunchKernelGGL(permute_cols_kernel, dim3(nblk((long long)K[b] * K[b], 256)), dim3(256),
+                         0, st, Vc[b], Vo[b], pb, one, K[b], K[b]);
+      std::swap(Wc[b], Wo[b]);
+      std::swap(Vc[b], Vo[b]);
+      desc[b].W = Wc[b];
+      desc[b].V = Vc[b];
+    }
+    SU2_HIP(hipMemcpyAsync(ddesc.p, desc.data(), sizeof(BlockDesc) * nb, hipMemcpyHostToDevice,
+                           st));
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(nblk(nb, 256)), dim3(256), 0, st,
+                       (unsigned long long*)doff.p, nb);
+    for (size_t r = 0; r < rounds.size(); ++r) {
+      const int nj = round_off[r + 1] - round_off[r];
+      if (nj == 0) continue;
+      hipLaunchKernelGGL(jacobi_pair_kernel, dim3(nj), dim3(NT), 0, st, (const BlockDesc*)ddesc.p,
+                         (const PairJob*)djobs.p + round_off[r], tol, 1e-14,
+                         (unsigned long long*)doff.p);
+    }
+    SU2_HIP(hipGetLastError());
+    SU2_HIP(hipMemcpyAsync(hoff.data(), doff.p, sizeof(unsigned long long) * nb,
+                           hipMemcpyDeviceToHost, st));
+    SU2_HIP(hipStreamSynchronize(st));
+    converged = true;
+    worst = 0;
+    for (int b = 0; b < nb; ++b) {
+      double v;
+      std::memcpy(&v, &hoff[b], sizeof v);
+      if (v >= 2.0) {   // some pair still rotated this sweep
+        converged = false;
+        worst = std::max(worst, v - 2.0);
+      }
+    }
+  }
+  // out of sweeps: still accept columns orthogonal to 1e-10 of their norms
+  SU2_CHECK(converged || worst <= 1e-10, kErrNumeric,
+            "jacobi svd: no convergence after %d sweeps (worst |w_i.w_j|/|w_i||w_j| = %.2e)",
+            max_sweeps, worst);
+
+  // singular values in ascending order, sign convention on V (W follows),
+  // results into the caller's buffers
+  norms();
+  for (int b = 0; b < nb; ++b) {
+    JacobiBlock& J = blocks[b];
+    const int k = K[b], l = L[b];
+    const double* w2 = hn.data() + noff[b];
+    std::vector<int> perm(k);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return w2[x] < w2[y]; });
+    J.s.resize(k);
+    for (int i = 0; i < k; ++i) J.s[i] = std::sqrt(w2[perm[i]]);
+    DevMem dpb(sizeof(int) * k), dsign(sizeof(double) * k), did(sizeof(int) * k);
+    std::vector<int> ident(k);
+    std::iota(ident.begin(), ident.end(), 0);
+    SU2_HIP(hipMemcpyAsync(dpb.p, perm.data(), sizeof(int) * k, hipMemcpyHostToDevice, st));
+    SU2_HIP(hipMemcpyAsync(did.p, ident.data(), sizeof(int) * k, hipMemcpyHostToDevice, st));
+    const double* one = (const double*)done.p + noff[b];
+    // V: sorted copy in the spare buffer, then the signs, then into J.V
+    hipLaunchKernelGGL(permute_cols_kernel, dim3(nblk((long long)k * k, 256)), dim3(256), 0, st,
+                       Vc[b], Vo[b], (const int*)dpb.p, one, k, k);
+    hipLaunchKernelGGL(col_sign_kernel, dim3(k), dim3(256), 0, st, (const double*)Vo[b], k,
+                       (double*)dsign.p);
+    hipLaunchKernelGGL(permute_cols_kernel, dim3(nblk((long long)k * k, 256)), dim3(256), 0, st,
+                       Vo[b], J.V == Vo[b] ? Vc[b] : J.V, (const int*)did.p,
+                       (const double*)dsign.p, k, k);
+    // W: same permutation and signs
+    double* wdst = J.W == Wc[b] ? Wo[b] : J.W;
+    hipLaunchKernelGGL(permute_cols_kernel, dim3(nblk((long long)l * k, 256)), dim3(256), 0, st,
+                       Wc[b], wdst, (const int*)dpb.p, (const double*)dsign.p, l, k);
+    if (wdst != J.W)
+      SU2_HIP(hipMemcpyAsync(J.W, wdst, sizeof(double) * (size_t)l * k, hipMemcpyDeviceToDevice,
+                             st));
+    if (J.V == Vo[b])
+      SU2_HIP(hipMemcpyAsync(J.V, Vc[b], sizeof(double) * (size_t)k * k, hipMemcpyDeviceToDevice,
+                             st));
+    SU2_HIP(hipStreamSynchronize(st));
+  }
+  SU2_HIP(hipGetLastError());
+}
+
+}  // namespace su2

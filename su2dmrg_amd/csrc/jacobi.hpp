@@ -1,0 +1,46 @@
+// One-sided block Jacobi SVD (Hestenes) of many dense blocks at once.
+//
+// The reference obtains each sector's singular vectors from the Gram matrix
+// with dsyev and then re-diagonalises the small-eigenvalue subspace again and
+// again to recover their accuracy (tensor.cpp:1472-1529).  Here the columns
+// of the block itself are rotated until mutually orthogonal: for a block A
+// (m x n) with k = min(m, n) the working matrix W is A (m >= n) or A^T, and
+// the accumulated rotation V (k x k) converges to the Gram eigenvectors, so
+// that W = A V holds the reference's u = A v and the column norms of W are
+// the singular values, accurate relative to each value rather than to the
+// largest one.
+//
+// Columns are cut into blocks of 32.  A sweep visits every pair of blocks
+// once, in round-robin rounds of disjoint pairs; one kernel launch runs all
+// pairs of a round for every block of the batch.  A workgroup per pair forms
+// the pair's Gram matrix (up to 64 x 64) over all rows, diagonalises it in LDS
+// by cyclic Jacobi and applies the eigenvector matrix to the pair's columns of
+// W and V.  Sweeps stop when every pair is orthogonal to
+// |g_ij| <= max(tol * sqrt(g_ii g_jj), 1e-15 ||A||_F^2) (the absolute floor
+// ends the iteration on columns that are rounding noise); blocks of up to 32
+// columns are one self-pair per sweep.
+#pragma once
+
+#include <vector>
+
+namespace su2 {
+
+class Context;
+
+struct JacobiBlock {
+  const double* A;   // m x n, column-major, leading dimension m (device)
+  int m, n;
+  double* W;         // out: L x k (L = max(m, n), k = min(m, n)), ld L, columns by
+                     //      ascending singular value (device, caller-allocated)
+  double* V;         // out: k x k, ld k, matching columns (device, caller-allocated)
+  std::vector<double> s;   // out: k singular values, ascending
+};
+
+// Decomposes every block; throws kErrNumeric when max_sweeps pass without
+// convergence.  W and V columns are sorted ascending by singular value; each
+// column of V has its first entry with |v| >= max|v|/2 made positive (the
+// LAPACK routine the reference calls leaves signs open), W's columns follow.
+void jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol = 1e-13,
+                        int max_sweeps = 40);
+
+}  // namespace su2

@@ -6,7 +6,7 @@
 #include "blas1.hpp"
 #include "common.hpp"
 #include "context.hpp"
-#include "eig.hpp"
+#include "jacobi.hpp"
 #include "ops.hpp"
 
 namespace su2 {
@@ -35,67 +35,6 @@ struct DevBuf {
   }
 };
 
-// Eigenpairs of the k x k Gram G (device, destroyed) with the reference's
-// refinement of the small-eigenvalue subspace.  V (k x k, ld k) receives the
-// eigenvectors, w the eigenvalues (ascending).
-void gram_eig(Context& c, double* G, int k, double* V, std::vector<double>& w) {
-  DevBuf Gc((long long)k * k);
-  dev_copy(c, Gc.p, G, (long long)k * k);
-  sym_eig(c, G, k, V, w);
-  int npp = 0;
-  for (int i = k - 1; i >= 0; --i)
-    if (w[i] / w[k - 1] < 0.1) {
-      npp = i + 1;
-      break;
-    }
-  if (npp <= 1) {
-    dev_fix_signs(c, V, k, k, k);
-    return;
-  }
-  // ap: current projected Gram (np x np), vp: its eigenvectors (np x np)
-  DevBuf ap((long long)k * k), vp((long long)k * k);
-  dev_copy(c, ap.p, Gc.p, (long long)k * k);
-  dev_copy(c, vp.p, V, (long long)k * k);
-  int np = k;
-  while (true) {
-    DevBuf X((long long)npp * np), app((long long)npp * npp), appc((long long)npp * npp);
-    DevBuf vpp((long long)npp * npp), T((long long)k * npp);
-    {
-      GemmBatch b;  // X = vp[:, :npp]^T ap
-      b.add(X.p, npp, npp, np, vp.p, np, 1, ap.p, np, 0, np);
-      b.run(c);
-    }
-    {
-      GemmBatch b;  // app = X vp[:, :npp]
-      b.add(app.p, npp, npp, npp, X.p, npp, 0, vp.p, np, 0, np);
-      b.run(c);
-    }
-    dev_copy(c, appc.p, app.p, (long long)npp * npp);
-    std::vector<double> ws;
-    sym_eig(c, app.p, npp, vpp.p, ws);
-    for (int i = 0; i < npp; ++i) w[i] = ws[i];
-    {
-      GemmBatch b;  // V[:, :npp] = V[:, :npp] vpp
-      b.add(T.p, k, k, npp, V, k, 0, vpp.p, npp, 0, npp);
-      b.run(c);
-    }
-    dev_copy(c, V, T.p, (long long)k * npp);
-    dev_copy(c, ap.p, appc.p, (long long)npp * npp);
-    dev_copy(c, vp.p, vpp.p, (long long)npp * npp);
-    np = npp;
-    int j = 0;
-    for (int i = npp - 1; i >= 0; --i)
-      if (ws[i] / ws[npp - 1] < 0.1) {
-        j = i + 1;
-        break;
-      }
-    npp = j;
-    if (npp <= 1) break;
-  }
-  dev_fix_signs(c, V, k, k, k);
-  SU2_HIP(hipStreamSynchronize(c.stream()));
-}
-
 }  // namespace
 
 SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int max_dcut) {
@@ -106,7 +45,7 @@ SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int
   struct Sec {
     int twoj, m, n, k, dc;
     const double* A;
-    DevBuf G, V, U;
+    DevBuf V, U;
     std::vector<double> s;   // k values, ascending, zeros for cut
   };
   std::vector<Sec> secs;
@@ -120,39 +59,38 @@ SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int
     s.A = psi.data() + b.offset;
     secs.push_back(std::move(s));
   }
-  // Gram matrices of all sectors in one grouped launch
-  {
-    GemmBatch g;
-    for (auto& s : secs) {
-      if (s.k == 0) continue;
-      s.G = DevBuf((long long)s.k * s.k);
-      if (s.m >= s.n) g.add(s.G.p, s.n, s.n, s.n, s.A, s.m, 1, s.A, s.m, 0, s.m);
-      else g.add(s.G.p, s.m, s.m, s.m, s.A, s.m, 0, s.A, s.m, 1, s.n);
-    }
-    g.run(c);
-  }
+  // Singular triplets of every sector in one batched one-sided Jacobi SVD.
+  // The reference diagonalises the Gram matrix of the short side and
+  // re-diagonalises its small-eigenvalue subspace until accurate
+  // (tensor.cpp:1472-1529); rotating the sector's own columns reaches the same
+  // triplets without forming the Gram matrix.  "U" receives the long side
+  // (A V for m >= n, A^T U for m < n: the reference's u = A v), "V" the short
+  // side's eigenvectors, both in ascending order of the singular values.
   std::vector<std::vector<double>> wv(secs.size());
-  for (size_t i = 0; i < secs.size(); ++i) {
-    Sec& s = secs[i];
-    if (s.k == 0) continue;
-    s.V = DevBuf((long long)s.k * s.k);
-    gram_eig(c, s.G.p, s.k, s.V.p, wv[i]);
-  }
-  // the other factor: U = A V (m >= n) or V = A^T U (m < n); both stored as
-  // "X" (the long side) with the eigenvectors in "E" (the short side)
   {
-    GemmBatch g;
-    for (auto& s : secs) {
+    std::vector<JacobiBlock> jb;
+    std::vector<size_t> idx;
+    for (size_t i = 0; i < secs.size(); ++i) {
+      Sec& s = secs[i];
       if (s.k == 0) continue;
-      if (s.m >= s.n) {
-        s.U = DevBuf((long long)s.m * s.k);
-        g.add(s.U.p, s.m, s.m, s.k, s.A, s.m, 0, s.V.p, s.k, 0, s.k);
-      } else {
-        s.U = DevBuf((long long)s.n * s.k);
-        g.add(s.U.p, s.n, s.n, s.k, s.A, s.m, 1, s.V.p, s.k, 0, s.k);
-      }
+      const int L = std::max(s.m, s.n);
+      s.U = DevBuf((long long)L * s.k);
+      s.V = DevBuf((long long)s.k * s.k);
+      JacobiBlock b;
+      b.A = s.A;
+      b.m = s.m;
+      b.n = s.n;
+      b.W = s.U.p;
+      b.V = s.V.p;
+      jb.push_back(b);
+      idx.push_back(i);
     }
-    g.run(c);
+    jacobi_svd_batched(c, jb);
+    for (size_t t = 0; t < jb.size(); ++t) {
+      std::vector<double>& w = wv[idx[t]];
+      w.resize(jb[t].s.size());
+      for (size_t q = 0; q < w.size(); ++q) w[q] = jb[t].s[q] * jb[t].s[q];
+    }
   }
   // singular values and column scaling (tensor.cpp:1533-1568, 1620-1655)
   std::vector<double> wsort;
@@ -175,12 +113,18 @@ SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int
     dc = std::min(dc, max_dcut);
     dc = std::min(dc, s.m >= s.n ? s.m : s.n);
     s.dc = dc;
-    // factors for the long-side matrix (held in U) and the eigenvectors (V)
+    // factors for the long-side matrix (held in U) and the eigenvectors (V).
+    // A kept value at or below 1e-11 of the sector's largest still counts as a
+    // multiplet (the reference's rule), but its columns are zeroed: their
+    // directions are rounding noise, and normalising noise (the s^-1 factor
+    // of an isometric side) would put near-duplicate states into the basis.
+    // Such a term carries weight s^2 <= 1e-22 of the sector's.
     std::vector<double> fl(k, 0.0), fe(k, 0.0);
     const double pl = s.m >= s.n ? p1 - 1.0 : p2 - 1.0;  // long side: U (m>=n) or V (m<n)
     const double pe = s.m >= s.n ? p2 : p1;
+    const double noise = 1e-11 * s.s[k - 1];
     for (int t = k - dc; t < k; ++t) {
-      if (s.s[t] > 0) {
+      if (s.s[t] > noise) {
         fl[t] = std::pow(s.s[t], pl);
         fe[t] = std::pow(s.s[t], pe);
       }
@@ -217,6 +161,11 @@ SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int
     }
   res.bond = make_leg(bt, bd, -1);
   res.discarded_weight = disc;
+  for (size_t i = 0; i < secs.size(); ++i)
+    for (int t = secs[i].k - keep[i]; t < secs[i].k; ++t) res.wout.push_back(secs[i].s[t]);
+  for (size_t i = 0; i < secs.size(); ++i)
+    for (int t = secs[i].k - secs[i].dc; t < secs[i].k - keep[i]; ++t)
+      res.wout.push_back(secs[i].s[t]);
   if (bt.empty()) fail(kErrNumeric, "svd: every singular value was cut");
   Leg la = psi.leg(0), lb = psi.leg(1);
   la.dir = 1;

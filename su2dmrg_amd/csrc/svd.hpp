@@ -26,6 +26,9 @@ struct SvdResult {
   std::unique_ptr<Tensor> V;             // [psi leg 1, bond]
   std::vector<std::vector<double>> kept; // ascending singular values per bond sector
   double discarded_weight = 0;           // sum of squares of the cut values
+  // the reference's wout (tensor_su2.cpp:1707-1731): kept values sector by
+  // sector, then the cut ones sector by sector, each ascending
+  std::vector<double> wout;
 };
 
 SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int max_dcut);

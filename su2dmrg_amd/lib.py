@@ -68,6 +68,8 @@ _SIG = {
     "su2_dmrg_energies": (_i, [_p, _dp, _i, _ip]),
     "su2_dmrg_history": (_i, [_p, _ip, _dp, _ip, _i, _ip]),
     "su2_dmrg_set_verbose": (_i, [_p, _i]),
+    "su2_dmrg_save": (_i, [_p, _p, C.c_char_p, _i, _i]),
+    "su2_dmrg_load": (_i, [_p, _p, C.c_char_p, _i, _i, _i, _dp]),
     "su2_dmrg_stats": (_i, [_p, _dp, _i]),
 }
 
@@ -419,6 +421,17 @@ class Dmrg:
 
     def set_max_dcut(self, d):
         self.ctx.check(lib.su2_dmrg_set_max_dcut(self.ctx.h, self.h, int(d)))
+
+    def save(self, directory, ky=0, exci=0):
+        """Checkpoint files in the reference's format (uu-*, ww-*, enr-*)."""
+        self.ctx.check(lib.su2_dmrg_save(self.ctx.h, self.h, os.fsencode(directory), ky, exci))
+
+    def load(self, directory, file_dcut=0, ky=0, exci=0):
+        """Restart from checkpoint files; returns the saved energy."""
+        e = _d()
+        self.ctx.check(lib.su2_dmrg_load(self.ctx.h, self.h, os.fsencode(directory),
+                                         int(file_dcut), ky, exci, C.byref(e)))
+        return e.value
 
     def set_verbose(self, level=1):
         self.ctx.check(lib.su2_dmrg_set_verbose(self.h, int(level)))

@@ -66,6 +66,36 @@ def test_truncated_svd_matches_oracle(ctx, p1, p2, dcut):
         np.testing.assert_allclose(W.T @ W, np.eye(W.shape[1]), atol=1e-11)
 
 
+@pytest.mark.parametrize("m,n,decades", [(300, 200, 20), (180, 420, 12), (520, 500, 6)])
+def test_svd_wide_spectrum_against_lapack(ctx, m, n, decades):
+    # Singular values spread over many decades, as in a converged DMRG
+    # wavefunction: every value to 1e-12 of the largest (the Gram route of the
+    # reference reaches only eps s_max^2 / s), the reconstruction to the
+    # weight of the columns zeroed at s <= 1e-11 s_max, and the left vectors
+    # orthogonal to 1e-13 s_max / min(s_i, s_j) where s > 1e-9 s_max.
+    from su2dmrg_amd import lib as L
+    _say(f"svd wide spectrum {m}x{n}, {decades} decades")
+    rng = np.random.default_rng(m + n)
+    k = min(m, n)
+    u, _ = np.linalg.qr(rng.standard_normal((m, k)))
+    v, _ = np.linalg.qr(rng.standard_normal((n, k)))
+    A = (u * np.logspace(0, -decades, k)) @ v.T
+    t = L.wavefunction(ctx, [1], [m], [n], {1: A})
+    U, V, kept, disc = L.svd_truncate(ctx, t, 0, 1, 100000)
+    ref = np.sort(np.linalg.svd(A, compute_uv=False))
+    s = np.array(kept)
+    assert len(s) == k
+    np.testing.assert_allclose(s, ref, rtol=0, atol=1e-12 * ref[-1])
+    # p1 = 0, p2 = 1: U orthonormal, V carries s
+    cut = s[s <= 1e-11 * s[-1]].sum()
+    np.testing.assert_allclose(U[1] @ V[1].T, A, rtol=0, atol=1e-13 * ref[-1] + cut)
+    big = s > 1e-9 * s[-1]
+    Ub = U[1][:, big]
+    sb = s[big]
+    bound = 1e-13 * s[-1] / np.minimum.outer(sb, sb) + 1e-13
+    assert np.all(np.abs(Ub.T @ Ub - np.eye(big.sum())) <= bound)
+
+
 def _run_both(ctx, lx, ly, jcoup, bdry, D, nsweep, totspin=0):
     from su2dmrg_amd import lib as L
     lat = od.Lattice(lx, ly, jcoup=jcoup, bdry=bdry)
@@ -102,9 +132,14 @@ def test_dmrg_cylinder_converges_to_ed(ctx, lx, ly, jcoup, twoS):
     lat = od.Lattice(lx, ly, jcoup=jcoup, bdry=2)
     e_ed = ed.ground_energy(lat, twoS)
     g = L.Dmrg(ctx, lx, ly, jcoup, 0.0, 2, twoS, 100)
-    g.run(nsweep=2, stop=None)
+    g.run(nsweep=6, stop=None)
     e = g.energies()
     assert abs(e[-1] - e_ed) < 1e-10 * abs(e_ed)
+    # variational: no sweep step (each treats the whole lattice; 2 (ns - 6)
+    # steps per sweep with nfree = 2) lies below the exact ground state
+    nstep = 6 * 2 * (lx * ly - 6)
+    if nstep:
+        assert min(e[-nstep:]) > e_ed - 1e-10 * abs(e_ed)
 
 
 def test_dmrg_config1_4x4_d200(ctx):
