@@ -475,6 +475,78 @@ def _bond_of(P):
     return Bond(list(P), [P[j].shape[1] for j in P], 1, sort=False)
 
 
+class LowerState:
+    """A converged lower state for excited-state targeting (exci > 0).
+
+    Its site tensors are kept once as blocks T[k][(j_a, j_b)] between the bond
+    left of site k (beta[k-1]) and the bond right of it (beta[k]); the
+    reference holds them as orth[j][k] and re-labels them between the left and
+    right forms (tensor_su2.cpp:749-836), which for spin-1/2 sites is a block
+    transpose (the re-coupling factor is 1).  `energy` is the Ritz value the
+    penalty uses (gs_enr[j], read back with the 1000 added, dmrg_su2_omp.cpp:
+    1100-1152).
+    """
+
+    def __init__(self, ns, beta, T, energy):
+        self.ns = ns
+        self.beta = beta          # k -> Bond, k = -1 .. ns-1
+        self.T = T                # k -> {(ja, jb): [d(ja) x d(jb)]}
+        self.energy = energy
+
+    @classmethod
+    def from_dmrg(cls, dm):
+        """The state a finished oracle run holds (left sites U, right sites V)."""
+        ns = dm.ns
+        beta, T = {}, {}
+        for k in range(-1, ns // 2):
+            beta[k] = dm.lb[k]
+        for k in range(ns // 2, ns + 1):
+            beta[k - 1] = dm.rb[k]
+        for k in range(ns):
+            T[k] = {}
+            if k < ns // 2:
+                E = Enlarged(dm.lb[k - 1], left=True)
+                for jb, P in dm.U[k].items():
+                    for ja, o, d in E.parts(jb):
+                        T[k][(ja, jb)] = P[o:o + d, :].copy()
+            else:
+                F = Enlarged(dm.rb[k + 1], left=False)
+                for ja, P in dm.V[k].items():
+                    for jb, o, d in F.parts(ja):
+                        T[k][(ja, jb)] = P[o:o + d, :].T.copy()
+        return cls(ns, beta, T, dm.energies[-1])
+
+    def left_view(self, k):
+        """Projector E = fuse(beta[k-1], site) -> beta[k] as {jb: [D_E x d]}."""
+        E = Enlarged(self.beta[k - 1], left=True)
+        out = {}
+        for jb, db in zip(self.beta[k].twoj, self.beta[k].dims):
+            if jb not in E.dims:
+                continue
+            P = np.zeros((E.dims[jb], db))
+            for ja, o, d in E.parts(jb):
+                blk = self.T[k].get((ja, jb))
+                if blk is not None:
+                    P[o:o + d, :] = blk
+            out[jb] = P
+        return E, out
+
+    def right_view(self, k):
+        """Projector F = fuse(site, beta[k]) -> beta[k-1] as {ja: [D_F x d]}."""
+        F = Enlarged(self.beta[k], left=False)
+        out = {}
+        for ja, da in zip(self.beta[k - 1].twoj, self.beta[k - 1].dims):
+            if ja not in F.dims:
+                continue
+            P = np.zeros((F.dims[ja], da))
+            for jb, o, d in F.parts(ja):
+                blk = self.T[k].get((ja, jb))
+                if blk is not None:
+                    P[o:o + d, :] = blk.T
+            out[ja] = P
+        return F, out
+
+
 class DMRG:
     """MPS projectors + environments for one lattice and target spin 2S.
 
@@ -484,7 +556,7 @@ class DMRG:
     Lenv[i]/Renv[i]: (H, {k: S_k}) in lb[i]/rb[i].
     """
 
-    def __init__(self, lat: Lattice, totspin=0, max_dcut=200, verbose=False):
+    def __init__(self, lat: Lattice, totspin=0, max_dcut=200, verbose=False, lower=()):
         self.lat = lat
         self.ns = lat.ns
         self.S = totspin
@@ -500,6 +572,11 @@ class DMRG:
         self.nmatvec = 0
         self.psi = None
         self.center = None
+        # excited states: overlap environments with each lower state
+        # (ovlp[j][i] of dmrg_su2_omp.cpp:319-601), {j_lower x j_current}
+        self.lower = list(lower)
+        self.OL = [{-1: {0: np.ones((1, 1))}} for _ in self.lower]
+        self.OR = [{self.ns: {totspin: np.ones((1, 1))}} for _ in self.lower]
 
     # ---- environments (dmrg_su2_omp.cpp:319-825) --------------------------
     def enlarged_left(self, i):
@@ -529,6 +606,24 @@ class DMRG:
         self.U[i] = U
         self.lb[i] = _bond_of(U)
         self.Lenv[i] = project(U, Hh, Sh)
+        for j, st in enumerate(self.lower):
+            # overlap_transformation (tensor_su2_dmrg_src.cpp:531-688): the
+            # lower state's site tensor against this run's, through ovlp[i-1]
+            Ej, Pj = st.left_view(i)
+            prev = self.OL[j][i - 1]
+            out = {}
+            for jb, P in U.items():
+                if jb not in Pj:
+                    continue
+                M = np.zeros((Pj[jb].shape[1], P.shape[1]))
+                for ja, o, d in E.parts(jb):
+                    if ja not in prev or (ja, jb) not in Ej.off:
+                        continue
+                    oj = Ej.off[(ja, jb)]
+                    dj = prev[ja].shape[0]
+                    M += Pj[jb][oj:oj + dj, :].T @ prev[ja] @ P[o:o + d, :]
+                out[jb] = M
+            self.OL[j][i] = out
 
     def push_right(self, i, V=None):
         E, Hh, Sh = self.enlarged_right(i)
@@ -537,6 +632,22 @@ class DMRG:
         self.V[i] = V
         self.rb[i] = _bond_of(V)
         self.Renv[i] = project(V, Hh, Sh)
+        for j, st in enumerate(self.lower):
+            Fj, Pj = st.right_view(i)
+            prev = self.OR[j][i + 1]
+            out = {}
+            for jr, P in V.items():
+                if jr not in Pj:
+                    continue
+                M = np.zeros((Pj[jr].shape[1], P.shape[1]))
+                for jq, o, d in E.parts(jr):
+                    if jq not in prev or (jq, jr) not in Fj.off:
+                        continue
+                    oj = Fj.off[(jq, jr)]
+                    dj = prev[jq].shape[0]
+                    M += Pj[jr][oj:oj + dj, :].T @ prev[jq] @ P[o:o + d, :]
+                out[jr] = M
+            self.OR[j][i] = out
 
     # ---- one two-site solve ---------------------------------------------
     def terms(self, il, ir, Sl, Sr):
@@ -576,8 +687,15 @@ class DMRG:
         terms = self.terms(il, ir, Sl, Sr)
         self.last_terms = (Hl, Hr, terms)
 
+        phis = self.lower_vectors(il, ir, El, Er, lay) if il + 1 == ir else []
+
         def apply(v):
-            return heff.heff_apply(v, Hl, Hr, terms)
+            out = heff.heff_apply(v, Hl, Hr, terms)
+            # - sum_j E_j <phi_j|v> phi_j (dmrg_su2.cpp:611-623)
+            for e_j, phi in phis:
+                c = -e_j * heff.inner(phi, v)
+                out = heff.axpy(c, phi, out)
+            return out
 
         nele = sum(a * b for a, b in lay.values())
         e, psi, nmv = lanczos(apply, psi0, mlanc=min(MLANC, nele + 1))
@@ -593,6 +711,34 @@ class DMRG:
         if self.verbose:
             print(f"step {il},{ir} E={e:.12f} D={sum(bond.dims)}")
         return e, U, V, bond, El, Er
+
+    def lower_vectors(self, il, ir, El, Er, lay):
+        """Each lower state in the current two-site basis (overlapvec of
+        prepare_input_vector, dmrg_su2_omp.cpp:229-254): its tensors at il and
+        ir mapped through the overlap environments, contracted on their bond."""
+        out = []
+        for j, st in enumerate(self.lower):
+            Ej, A = st.left_view(il)
+            Fj, B = st.right_view(ir)
+            L, R = self.OL[j][il - 1], self.OR[j][ir + 1]
+            phi = {}
+            for jf, (da, db) in lay.items():
+                if jf not in A or jf not in B:
+                    phi[jf] = np.zeros((da, db))
+                    continue
+                At = np.zeros((da, A[jf].shape[1]))
+                for ja, o, d in El.parts(jf):
+                    if ja in L and (ja, jf) in Ej.off:
+                        oj = Ej.off[(ja, jf)]
+                        At[o:o + d, :] = L[ja].T @ A[jf][oj:oj + L[ja].shape[0], :]
+                Bt = np.zeros((db, B[jf].shape[1]))
+                for jq, o, d in Er.parts(jf):
+                    if jq in R and (jq, jf) in Fj.off:
+                        oj = Fj.off[(jq, jf)]
+                        Bt[o:o + d, :] = R[jq].T @ B[jf][oj:oj + R[jq].shape[0], :]
+                phi[jf] = At @ Bt.T
+            out.append((st.energy, phi))
+        return out
 
     # ---- Lanczos start vectors carried across one site --------------------
     def start_left(self, il, A, b):

@@ -75,3 +75,37 @@ def ground_energy(lat, twoS=0, k=6):
         if abs(_s2(ns, states, vec) - target) < 1e-6:
             return float(e)
     raise RuntimeError("no level with the requested total spin among the lowest")
+
+
+def sector_energies(lat, twoS=0, count=2, k=16):
+    """The `count` lowest energies with total spin S = twoS/2 (multiplets of
+    that spin only), for excited-state checks.  Degenerate levels of different
+    spin are separated by diagonalising S^2 inside each degenerate group."""
+    ns = lat.ns
+    states = _basis(ns, twoS)
+    H = _heisenberg(ns, lat.bonds(), states)
+    n = H.shape[0]
+    if n <= 600:
+        w, v = np.linalg.eigh(H.toarray())
+    else:
+        w, v = spla.eigsh(H, k=min(k, n - 2), which="SA", tol=1e-13)
+        order = np.argsort(w)
+        w, v = w[order], v[:, order]
+    bonds = [(i, j, 2.0) for i in range(ns) for j in range(i + 1, ns)]
+    S2op = _heisenberg(ns, bonds, states)
+    target = 0.25 * twoS * (twoS + 2)
+    out = []
+    i = 0
+    while i < len(w) and len(out) < count:
+        grp = [i]
+        while grp[-1] + 1 < len(w) and abs(w[grp[-1] + 1] - w[i]) < 1e-9 * max(1.0, abs(w[i])):
+            grp.append(grp[-1] + 1)
+        V = v[:, grp]
+        s2 = V.T @ (S2op @ V) + 0.75 * ns * np.eye(len(grp))
+        for val in np.linalg.eigvalsh(0.5 * (s2 + s2.T)):
+            if abs(val - target) < 1e-6 and len(out) < count:
+                out.append(float(w[i]))
+        i = grp[-1] + 1
+    if len(out) < count:
+        raise RuntimeError("not enough levels with the requested total spin among the lowest")
+    return out

@@ -228,3 +228,27 @@ def load(dm, directory, dcut=None, ky=0, exci=0):
     with open(os.path.join(directory, base_name("enr", ns, D, S, ky, exci) + ".dat")) as f:
         vals = [float(x) for x in f.read().split()]
     return vals[-1] + 1000
+
+
+def lower_state(directory, ns, dcut, twoS, exci, ky=0):
+    """A lower state read from its checkpoint files (read_mps into orth[exci],
+    dmrg_su2_omp.cpp:944-1068; energy from read_enr, 1100-1152)."""
+    from .dmrg import LowerState
+    base = os.path.join(directory, base_name("uu", ns, dcut, twoS, ky, exci))
+    beta, T = {}, {}
+    for k in range(ns):
+        legs, blocks = read_tensor3(f"{base}-{k}")
+        T[k] = {}
+        if k < ns // 2:
+            beta[k - 1] = Bond(legs[0].twoj, legs[0].dims, 1, sort=False)
+            beta[k] = Bond(legs[2].twoj, legs[2].dims, 1, sort=False)
+            for (ja, _, jb), b in blocks.items():
+                T[k][(ja, jb)] = b[:, 0, :]
+        else:
+            beta[k] = Bond(legs[1].twoj, legs[1].dims, 1, sort=False)
+            beta[k - 1] = Bond(legs[2].twoj, legs[2].dims, 1, sort=False)
+            for (_, jq, jr), b in blocks.items():
+                T[k][(jr, jq)] = b[0, :, :].T
+    with open(os.path.join(directory, base_name("enr", ns, dcut, twoS, ky, exci) + ".dat")) as f:
+        energy = [float(x) for x in f.read().split()][-1] + 1000
+    return LowerState(ns, beta, T, energy)
