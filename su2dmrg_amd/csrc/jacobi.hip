@@ -18,7 +18,9 @@ constexpr int CB = 32;          // columns per block
 constexpr int NP = 2 * CB;      // largest pair problem
 constexpr int RCH = 32;         // rows per staged chunk
 constexpr int NT = 256;         // threads per pair workgroup
-constexpr int INNER_SWEEPS = 40;
+constexpr int INNER_SWEEPS = 3; // inner Jacobi sweeps per visit of a pair
+constexpr int MAX_SLICES = 8;   // row slices per pair
+constexpr int SLICE_TARGET = 256;   // workgroups a round aims for (one per CU)
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
@@ -33,6 +35,7 @@ struct PairJob {
   int blk;          // index into the block table
   int p0, bp;       // first column block
   int q0, bq;       // second column block (bq = 0: the first block alone)
+  int P;            // row slices (workgroups) that share the pair
 };
 
 __device__ __forceinline__ int rr_p(int r, int k, int m) { return k == 0 ? m - 1 : (r + k) % (m - 1); }
@@ -40,42 +43,40 @@ __device__ __forceinline__ int rr_q(int r, int k, int m) {
   return k == 0 ? r : (r - k + (m - 1)) % (m - 1);
 }
 
-// Orthogonalise the columns of one pair of column blocks of W: Gram matrix,
-// in-LDS cyclic Jacobi, rotation of the pair's columns of W and V.  Records
-// the pair's largest scaled off-diagonal Gram entry (before rotating) in
-// off[blk] (as ordered bits of a non-negative double).
-__global__ __launch_bounds__(NT) void jacobi_pair_kernel(const BlockDesc* __restrict__ blocks,
-                                                         const PairJob* __restrict__ jobs,
-                                                         double tol_rel, double tol_abs,
-                                                         unsigned long long* __restrict__ off) {
-  __shared__ double G[NP][NP + 1];
-  __shared__ double R[NP][NP + 1];
-  __shared__ double X[RCH][NP + 1];
-  __shared__ double cs[NP / 2], sn[NP / 2];
-  __shared__ int pp[NP / 2], qq[NP / 2];
-  __shared__ int colmap[NP];
-  __shared__ double red[NT];
-  __shared__ int nrot;
+// rows [lo, hi) of slice s when L rows are cut into P slices
+__device__ __forceinline__ void slice_rows(int L, int P, int s, int& lo, int& hi) {
+  const int per = (L + P - 1) / P;
+  lo = min(L, s * per);
+  hi = min(L, lo + per);
+}
 
-  const PairJob J = jobs[blockIdx.x];
+// Partial Gram matrix of one pair over one row slice of W, on the matrix
+// cores.  32-row chunks of the pair's columns are staged in LDS (columns >= n
+// zero); wave w accumulates the 16 x 16 tiles (w, 0..3) with
+// v_mfma_f64_16x16x4f64: A = X^T fragment (lane l: X[r + l/16][16 w + l%16]),
+// B = X fragment (lane l: X[r + l/16][16 j + l%16]), D lane l:
+// G[16 w + l/16 + 4 q][16 j + l%16].  The 64 x 64 result goes row-major to
+// gpart[blockIdx.x] (workgroup = job * pmax + slice).
+__global__ __launch_bounds__(NT) void gram_partial_kernel(const BlockDesc* __restrict__ blocks,
+                                                          const PairJob* __restrict__ jobs,
+                                                          int pmax, double* __restrict__ gpart) {
+  __shared__ double X[RCH][NP + 1];
+  __shared__ int colmap[NP];
+  const int job = blockIdx.x / pmax, slice = blockIdx.x - job * pmax;
+  const PairJob J = jobs[job];
+  if (slice >= J.P) return;
   const BlockDesc B = blocks[J.blk];
   const int n = J.bp + J.bq;
-  const int t = threadIdx.x;
-  if (n < 2) return;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   for (int c = t; c < n; c += NT) colmap[c] = c < J.bp ? J.p0 + c : J.q0 + (c - J.bp);
-
-  // ---- Gram matrix G = X^T X of the pair's columns on the matrix cores ----
-  // 32-row chunks of the columns are staged in LDS (columns >= n zero); wave
-  // w accumulates the 16 x 16 tiles (w, 0..3) with v_mfma_f64_16x16x4f64:
-  // A = X^T fragment (lane l: X[r + l/16][16 w + l%16]), B = X fragment
-  // (lane l: X[r + l/16][16 j + l%16]), D lane l: G[16 w + l/16 + 4 q][16 j + l%16].
-  const int lane = t & 63, wave = t >> 6;
+  int lo, hi;
+  slice_rows(B.L, J.P, slice, lo, hi);
   f64x4 acc[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) acc[j] = f64x4{0.0, 0.0, 0.0, 0.0};
   __syncthreads();
-  for (int r0 = 0; r0 < B.L; r0 += RCH) {
-    const int nr = min(RCH, B.L - r0);
+  for (int r0 = lo; r0 < hi; r0 += RCH) {
+    const int nr = min(RCH, hi - r0);
     for (int e = t; e < RCH * NP; e += NT) {
       const int rr = e % RCH, c = e / RCH;
       X[rr][c] = (rr < nr && c < n) ? B.W[(r0 + rr) + (long long)colmap[c] * B.L] : 0.0;
@@ -91,10 +92,63 @@ __global__ __launch_bounds__(NT) void jacobi_pair_kernel(const BlockDesc* __rest
     }
     __syncthreads();
   }
+  double* out = gpart + (size_t)blockIdx.x * NP * NP;
 #pragma unroll
   for (int j = 0; j < 4; ++j)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) G[16 * wave + (lane >> 4) + 4 * q][16 * j + (lane & 15)] = acc[j][q];
+    for (int q = 0; q < 4; ++q)
+      out[(16 * wave + (lane >> 4) + 4 * q) * NP + 16 * j + (lane & 15)] = acc[j][q];
+}
+
+// Orthogonalise one pair of column blocks: every row slice of the pair sums
+// the same partial Gram matrices in the same order, so the P workgroups of a
+// pair hold bit-identical Gram matrices, convergence decisions and rotations;
+// each then rotates its own rows of W and V.  Slice 0 records the pair's
+// largest scaled off-diagonal Gram entry (before rotating) in off[blk] (as
+// ordered bits of a non-negative double).
+//
+// The pair's Gram matrix is diagonalised in LDS by parallel cyclic Jacobi
+// (round-robin rounds of disjoint rotations, at most INNER_SWEEPS sweeps: the
+// outer iteration revisits the pair, and more inner sweeps do not lower the
+// outer sweep count).  A round updates G <- J^T G J by 2 x 2 blocks, one
+// thread per block pair, with the rotated entry set to zero and the diagonal
+// moved by t g_pq; a rotation is skipped below max(tol_in sqrt(g_pp g_qq),
+// 0.1 tol_abs ||A||_F (sqrt g_pp + sqrt g_qq)), ten times below the outer
+// test.
+__global__ __launch_bounds__(NT) void jacobi_rotate_kernel(const BlockDesc* __restrict__ blocks,
+                                                           const PairJob* __restrict__ jobs,
+                                                           int pmax,
+                                                           const double* __restrict__ gpart,
+                                                           double tol_rel, double tol_abs,
+                                                           unsigned long long* __restrict__ off) {
+  __shared__ double G[NP][NP + 1];
+  __shared__ double R[NP][NP + 1];
+  __shared__ double X[RCH][NP + 1];
+  __shared__ double cs[NP / 2], sn[NP / 2], tn[NP / 2];
+  __shared__ int pp[NP / 2], qq[NP / 2];
+  __shared__ int colmap[NP];
+  __shared__ double red[NT];
+  __shared__ int rotated[INNER_SWEEPS];
+
+  const int job = blockIdx.x / pmax, slice = blockIdx.x - job * pmax;
+  const PairJob J = jobs[job];
+  if (slice >= J.P) return;
+  const BlockDesc B = blocks[J.blk];
+  const int n = J.bp + J.bq;
+  const int t = threadIdx.x;
+  if (n < 2) return;
+  for (int c = t; c < n; c += NT) colmap[c] = c < J.bp ? J.p0 + c : J.q0 + (c - J.bp);
+  if (t < INNER_SWEEPS) rotated[t] = 0;
+  {
+    const double* gp = gpart + (size_t)job * pmax * NP * NP;
+    for (int e = t; e < NP * NP; e += NT) {
+      const int a = e / NP, b = e % NP;
+      double g = 0.0;
+      for (int s = 0; s < J.P; ++s) g += gp[(size_t)s * NP * NP + e];
+      G[a][b] = g;
+      R[a][b] = a == b ? 1.0 : 0.0;
+    }
+  }
   __syncthreads();
 
   // ---- convergence measure of this pair ----
@@ -124,7 +178,7 @@ __global__ __launch_bounds__(NT) void jacobi_pair_kernel(const BlockDesc* __rest
     __syncthreads();
   }
   const double top = red[0];
-  if (t == 0) {
+  if (t == 0 && slice == 0) {
     const double ratio = top >= 4.0 ? top - 4.0 : top;
     // a pair that still needs a rotation reports at least 2 (above any ratio)
     const double rep = top >= 4.0 ? 2.0 + ratio : ratio;
@@ -132,79 +186,103 @@ __global__ __launch_bounds__(NT) void jacobi_pair_kernel(const BlockDesc* __rest
   }
   if (top < 4.0) return;   // orthogonal to tolerance: nothing to rotate
 
-  // ---- eigenvectors of G by cyclic Jacobi (rotations relative to the diagonal) ----
-  for (int e = t; e < NP * NP; e += NT) {
-    const int a = e % NP, b = e / NP;
-    R[a][b] = a == b ? 1.0 : 0.0;
-  }
-  const int m = n + (n & 1);
+  // ---- eigenvectors of G by parallel cyclic Jacobi ----
+  const int m = n + (n & 1);   // index n (odd n) is a zero row and column
   const int npair = m / 2;
-  __syncthreads();
+  const double tol_in = 1e-14;
+  const double fl = 0.1 * tol_abs * B.normF;
   for (int sweep = 0; sweep < INNER_SWEEPS; ++sweep) {
-    if (t == 0) nrot = 0;
-    __syncthreads();
     for (int r = 0; r < m - 1; ++r) {
       if (t < npair) {
         const int p = rr_p(r, t, m), q = rr_q(r, t, m);
-        double c = 1.0, s = 0.0;
+        double c = 1.0, s = 0.0, tt = 0.0;
         if (p < n && q < n) {
           const double apq = G[p][q];
           const double app = G[p][p], aqq = G[q][q];
-          if (fabs(apq) > 1e-15 * sqrt(fabs(app * aqq)) && fabs(apq) > 1e-300) {
+          const double thr = fmax(tol_in * sqrt(fabs(app * aqq)),
+                                  fl * (sqrt(fabs(app)) + sqrt(fabs(aqq))));
+          if (fabs(apq) > thr && fabs(apq) > 1e-300) {
             const double tau = (aqq - app) / (2.0 * apq);
-            const double tt = fabs(tau) > 1e150 ? 0.5 / tau
-                                                : (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            tt = fabs(tau) > 1e150 ? 0.5 / tau
+                                   : (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
             c = 1.0 / sqrt(1.0 + tt * tt);
             s = tt * c;
-            atomicAdd(&nrot, 1);
+            rotated[sweep] = 1;
           }
         }
         pp[t] = p;
         qq[t] = q;
         cs[t] = c;
         sn[t] = s;
+        tn[t] = tt;
       }
       __syncthreads();
-      for (int e = t; e < npair * n; e += NT) {   // G <- G J
-        const int k = e / n, row = e % n;
-        const int p = pp[k], q = qq[k];
-        if (p >= n || q >= n || sn[k] == 0.0) continue;
-        const double c = cs[k], s = sn[k];
-        const double x = G[row][p], y = G[row][q];
-        G[row][p] = c * x - s * y;
-        G[row][q] = s * x + c * y;
-      }
-      __syncthreads();
-      for (int e = t; e < npair * n; e += NT) {   // G <- J^T G, R <- R J
-        const int k = e / n, col = e % n;
-        const int p = pp[k], q = qq[k];
-        if (p >= n || q >= n || sn[k] == 0.0) continue;
-        const double c = cs[k], s = sn[k];
-        const double x = G[p][col], y = G[q][col];
-        G[p][col] = c * x - s * y;
-        G[q][col] = s * x + c * y;
-        const double u = R[col][p], v = R[col][q];
-        R[col][p] = c * u - s * v;
-        R[col][q] = s * u + c * v;
+      {
+        // G <- J^T G J on the 2 x 2 blocks (k, k2), k <= k2, mirrored to (k2, k)
+        const int k = t & 31;
+        if (k < npair) {
+          const int p = pp[k], q = qq[k];
+          const double c = cs[k], s = sn[k];
+          for (int k2 = t >> 5; k2 < npair; k2 += NT / 32) {
+            if (k2 < k) continue;
+            if (k2 == k) {
+              if (s == 0.0) continue;
+              const double apq = G[p][q];
+              G[p][p] -= tn[k] * apq;
+              G[q][q] += tn[k] * apq;
+              G[p][q] = 0.0;
+              G[q][p] = 0.0;
+              continue;
+            }
+            const double c2 = cs[k2], s2 = sn[k2];
+            if (s == 0.0 && s2 == 0.0) continue;
+            const int p2 = pp[k2], q2 = qq[k2];
+            const double m00 = G[p][p2], m01 = G[p][q2], m10 = G[q][p2], m11 = G[q][q2];
+            const double t00 = c2 * m00 - s2 * m01, t01 = s2 * m00 + c2 * m01;
+            const double t10 = c2 * m10 - s2 * m11, t11 = s2 * m10 + c2 * m11;
+            const double n00 = c * t00 - s * t10, n01 = c * t01 - s * t11;
+            const double n10 = s * t00 + c * t10, n11 = s * t01 + c * t11;
+            G[p][p2] = n00;
+            G[p][q2] = n01;
+            G[q][p2] = n10;
+            G[q][q2] = n11;
+            G[p2][p] = n00;
+            G[q2][p] = n01;
+            G[p2][q] = n10;
+            G[q2][q] = n11;
+          }
+        }
+        // R <- R J (rows >= n hold zeros in the rotated columns)
+        const int row = t & (NP - 1);
+        for (int kk = t / NP; kk < npair; kk += NT / NP) {
+          const double s = sn[kk];
+          if (s == 0.0) continue;
+          const double c = cs[kk];
+          const int p = pp[kk], q = qq[kk];
+          const double u = R[row][p], v = R[row][q];
+          R[row][p] = c * u - s * v;
+          R[row][q] = s * u + c * v;
+        }
       }
       __syncthreads();
     }
-    if (nrot == 0) break;
-    __syncthreads();
+    if (!rotated[sweep]) break;
   }
 
-  // ---- rotate the pair's columns: W[:, cols] <- W[:, cols] R, same for V ----
+  // ---- rotate this slice's rows: W[:, cols] <- W[:, cols] R, same for V ----
   // Y = X R per 32-row chunk on the matrix cores: wave w owns the row tile
   // w % 2 and the column tiles 2 (w / 2) and 2 (w / 2) + 1; A = X fragment
   // (lane l: X[16 rt + l%16][k + l/16]), B = R fragment (lane l: R[k + l/16][16 ct + l%16]).
   // The result goes back through LDS so each column segment is written whole.
+  const int lane = t & 63, wave = t >> 6;
   const int rt = wave & 1, ct0 = 2 * (wave >> 1);
   for (int which = 0; which < 2; ++which) {
     double* M = which == 0 ? B.W : B.V;
-    const int rows = which == 0 ? B.L : B.k;
     const long long ld = which == 0 ? B.L : B.k;
-    for (int r0 = 0; r0 < rows; r0 += RCH) {
-      const int nr = min(RCH, rows - r0);
+    int lo, hi;
+    slice_rows(which == 0 ? B.L : B.k, J.P, slice, lo, hi);
+    for (int r0 = lo; r0 < hi; r0 += RCH) {
+      const int nr = min(RCH, hi - r0);
       __syncthreads();
       for (int e = t; e < RCH * NP; e += NT) {
         const int rr = e % RCH, c = e / RCH;
@@ -399,7 +477,7 @@ void jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol
     auto width = [&](int i) { return i >= ncb ? 0 : std::min(CB, K[b] - i * CB); };
     if (ncb == 1) {
       if (rounds.empty()) rounds.resize(1);
-      if (K[b] >= 2) rounds[0].push_back(PairJob{b, 0, K[b], 0, 0});
+      if (K[b] >= 2) rounds[0].push_back(PairJob{b, 0, K[b], 0, 0, 1});
       continue;
     }
     const int mp = ncb + (ncb & 1);
@@ -410,16 +488,29 @@ void jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol
         int Q = (k == 0) ? r : (r - k + (mp - 1)) % (mp - 1);
         if (width(P) == 0) std::swap(P, Q);
         if (width(P) == 0) continue;
-        rounds[r].push_back(PairJob{b, P * CB, width(P), width(Q) ? Q * CB : 0, width(Q)});
+        rounds[r].push_back(PairJob{b, P * CB, width(P), width(Q) ? Q * CB : 0, width(Q), 1});
       }
   }
-  std::vector<int> round_off(rounds.size() + 1, 0);
+  // Row slices: a round with few pairs spreads each pair over up to
+  // MAX_SLICES workgroups (at least 64 rows each) so that it fills the chip.
+  std::vector<int> round_off(rounds.size() + 1, 0), round_pmax(rounds.size(), 1);
   std::vector<PairJob> all;
+  size_t gpart_max = 0;
   for (size_t r = 0; r < rounds.size(); ++r) {
     round_off[r] = (int)all.size();
+    const int nj = (int)rounds[r].size();
+    const int pmax = nj == 0 ? 1 : std::min(MAX_SLICES, std::max(1, (SLICE_TARGET + nj - 1) / nj));
+    int used = 1;
+    for (PairJob& j : rounds[r]) {
+      j.P = std::max(1, std::min(pmax, L[j.blk] / 64));
+      used = std::max(used, j.P);
+    }
+    round_pmax[r] = used;
+    gpart_max = std::max(gpart_max, (size_t)nj * used);
     all.insert(all.end(), rounds[r].begin(), rounds[r].end());
   }
   round_off[rounds.size()] = (int)all.size();
+  DevMem dgpart(sizeof(double) * NP * NP * std::max<size_t>(1, gpart_max));
   DevMem ddesc(sizeof(BlockDesc) * nb), djobs(sizeof(PairJob) * std::max<size_t>(1, all.size())),
       doff(sizeof(unsigned long long) * nb);
   if (!all.empty())
@@ -457,8 +548,12 @@ void jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol
     for (size_t r = 0; r < rounds.size(); ++r) {
       const int nj = round_off[r + 1] - round_off[r];
       if (nj == 0) continue;
-      hipLaunchKernelGGL(jacobi_pair_kernel, dim3(nj), dim3(NT), 0, st, (const BlockDesc*)ddesc.p,
-                         (const PairJob*)djobs.p + round_off[r], tol, 1e-14,
+      const int pm = round_pmax[r];
+      const PairJob* jr = (const PairJob*)djobs.p + round_off[r];
+      hipLaunchKernelGGL(gram_partial_kernel, dim3(nj * pm), dim3(NT), 0, st,
+                         (const BlockDesc*)ddesc.p, jr, pm, (double*)dgpart.p);
+      hipLaunchKernelGGL(jacobi_rotate_kernel, dim3(nj * pm), dim3(NT), 0, st,
+                         (const BlockDesc*)ddesc.p, jr, pm, (const double*)dgpart.p, tol, 1e-14,
                          (unsigned long long*)doff.p);
     }
     SU2_HIP(hipGetLastError());

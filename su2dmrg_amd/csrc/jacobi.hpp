@@ -11,14 +11,19 @@
 // largest one.
 //
 // Columns are cut into blocks of 32.  A sweep visits every pair of blocks
-// once, in round-robin rounds of disjoint pairs; one kernel launch runs all
-// pairs of a round for every block of the batch.  A workgroup per pair forms
-// the pair's Gram matrix (up to 64 x 64) over all rows, diagonalises it in LDS
-// by cyclic Jacobi and applies the eigenvector matrix to the pair's columns of
-// W and V.  Sweeps stop when every pair is orthogonal to
-// |g_ij| <= max(tol * sqrt(g_ii g_jj), 1e-15 ||A||_F^2) (the absolute floor
-// ends the iteration on columns that are rounding noise); blocks of up to 32
-// columns are one self-pair per sweep.
+// once, in round-robin rounds of disjoint pairs; two kernel launches run all
+// pairs of a round for every block of the batch.  The first forms partial
+// Gram matrices (up to 64 x 64) of each pair over row slices of W, the second
+// sums them, diagonalises the pair's Gram matrix in LDS by parallel cyclic
+// Jacobi (at most three inner sweeps per visit) and applies the eigenvector
+// matrix to the pair's columns of W and V, each slice to its own rows.  A
+// round with few pairs cuts every pair into up to 8 slices so that it spans
+// the chip.  Before each sweep the columns are put in descending norm order.
+// Sweeps stop when every pair is orthogonal to
+// |g_ij| <= max(tol * sqrt(g_ii g_jj), 1e-14 ||A||_F (sqrt g_ii + sqrt g_jj))
+// (the absolute floor ends the iteration on columns that carry rounding
+// errors of order eps ||A||); columns with norm <= 1e-13 ||A||_F are left
+// out of the test.  Blocks of up to 32 columns are one self-pair per sweep.
 #pragma once
 
 #include <vector>
