@@ -187,6 +187,17 @@ int su2_dmrg_save(su2_ctx* ctx, su2_dmrg* d, const char* dir, int ky, int exci);
  * error the previous state is kept. */
 int su2_dmrg_load(su2_ctx* ctx, su2_dmrg* d, const char* dir, int file_dcut, int ky, int exci,
                   double* energy);
+/* Excited-state targeting (the reference's -e exci > 0: orth[j] read by
+ * read_mps, dmrg_su2_omp.cpp:944-1068, its energy by read_enr, 1100-1152).
+ * Adds the state saved in dir as uu/enr-{ns}-{file_dcut}-{2S}-{ky}-{exci}-*
+ * (file_dcut 0: this run's max_dcut) to the states the run is kept
+ * orthogonal to: the run carries the overlap environments ovlp[j]
+ * (dmrg_su2_omp.cpp:327-328, 386, 470-471, 528) and every H_eff apply on
+ * adjacent sites adds -E_j <phi_j|psi> phi_j (dmrg_su2.cpp:611-623).  For the
+ * n-th excited state call it for exci = 0 .. n-1, before su2_dmrg_warmup (or
+ * after su2_dmrg_load), and save the result with exci = n. */
+int su2_dmrg_add_lower_state(su2_ctx* ctx, su2_dmrg* d, const char* dir, int file_dcut, int ky,
+                             int exci);
 /* level 1: one line per solve on stderr (step, sites, Ritz value, kept
  * multiplets, discarded weight, matvecs, seconds); the reference logs each
  * step to stdout (dmrg_su2_omp.cpp:113,129,145). */

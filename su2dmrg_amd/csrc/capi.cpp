@@ -499,6 +499,16 @@ int su2_dmrg_load(su2_ctx* ctx, su2_dmrg* d, const char* dir, int file_dcut, int
   });
 }
 
+int su2_dmrg_add_lower_state(su2_ctx* ctx, su2_dmrg* d, const char* dir, int file_dcut, int ky,
+                             int exci) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(d, "dmrg");
+    need(dir, "dir");
+    d->d.add_lower_state(dir, file_dcut, ky, exci);
+  });
+}
+
 int su2_dmrg_set_verbose(su2_dmrg* d, int level) {
   return guard(nullptr, [&] {
     need(d, "dmrg");

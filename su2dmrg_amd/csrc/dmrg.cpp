@@ -360,6 +360,9 @@ Dmrg::StepOut Dmrg::step(int il, int ir, std::unique_ptr<Tensor> psi0, double p1
   HeffPlan plan(c, *psi, El.H.get(), Er.H.get(), terms);
   if (var1_ >= 0) plan.set_variants(var1_, var2_);
   const int mlanc = (int)std::min<long long>(kMlanc, psi->size() + 1);
+  // lower states in this two-site basis (adjacent sites only, dmrg_su2.cpp:611)
+  std::vector<std::unique_ptr<Tensor>> phis;
+  if (!lower_.empty() && il + 1 == ir) phis = lower_vectors(il, ir, o.FL, o.FR, *psi);
   hipEvent_t e0, e1;
   SU2_HIP(hipEventCreate(&e0));
   SU2_HIP(hipEventCreate(&e1));
@@ -372,6 +375,11 @@ Dmrg::StepOut Dmrg::step(int il, int ir, std::unique_ptr<Tensor> psi0, double p1
     float ms = 0;
     SU2_HIP(hipEventElapsedTime(&ms, e0, e1));
     mv_ms += ms;
+    // - sum_j E_j <phi_j|x> phi_j (dmrg_su2.cpp:611-623)
+    for (size_t j = 0; j < phis.size(); ++j) {
+      const double ov = dev_dot(c, phis[j]->data(), x, psi->size());
+      dev_axpy(c, y, phis[j]->data(), psi->size(), -lower_[j]->energy * ov);
+    }
   };
   LanczosResult lr = lanczos_ground(c, apply, psi->data(), psi->size(), mlanc);
   (void)hipEventDestroy(e0);
@@ -534,6 +542,7 @@ void Dmrg::ensure_left(int i) {
   L_[i] = enlarged_left(i, E);     // untruncated: the projector is the identity
   lb_[i] = E.fused;
   U_[i] = identity_projector(c_, E.fused);
+  overlap_left(i);
 }
 
 void Dmrg::ensure_right(int i) {
@@ -542,6 +551,7 @@ void Dmrg::ensure_right(int i) {
   R_[i] = enlarged_right(i, E);
   rb_[i] = E.fused;
   V_[i] = identity_projector(c_, E.fused);
+  overlap_right(i);
 }
 
 // do_idmrg (dmrg_su2_omp.cpp:20-95)
@@ -561,11 +571,13 @@ double Dmrg::warmup() {
     rb_[ir] = o.bond;
     R_[ir] = project(*o.V, *o.Er);
     V_[ir] = std::move(o.V);
+    overlap_right(ir);
     if (final) {
       U_[il] = std::move(o.U);
     } else {
       L_[il] = project(*o.U, *o.El);
       U_[il] = std::move(o.U);
+      overlap_left(il);
     }
     last = o.energy;
   }
@@ -594,10 +606,12 @@ double Dmrg::sweep() {
     if (!turn) {
       R_[ir] = project(*o.V, *o.Er);
       V_[ir] = std::move(o.V);
+      overlap_right(ir);
       A = std::move(o.U);
     } else {
       L_[il] = project(*o.U, *o.El);
       U_[il] = std::move(o.U);
+      overlap_left(il);
       B = std::move(o.V);
     }
     e = o.energy;
@@ -614,10 +628,12 @@ double Dmrg::sweep() {
     if (!turn) {
       L_[il] = project(*o.U, *o.El);
       U_[il] = std::move(o.U);
+      overlap_left(il);
       B = std::move(o.V);
     } else {
       R_[ir] = project(*o.V, *o.Er);
       V_[ir] = std::move(o.V);
+      overlap_right(ir);
       A = std::move(o.U);
     }
     e = o.energy;
@@ -633,6 +649,7 @@ double Dmrg::sweep() {
     rb_[ir] = bond;
     R_[ir] = project(*o.V, *o.Er);
     V_[ir] = std::move(o.V);
+    overlap_right(ir);
     A = std::move(o.U);
     il_last = il;
     e = o.energy;

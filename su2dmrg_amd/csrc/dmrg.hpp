@@ -58,6 +58,22 @@ struct StepRecord {
 // fused sectors (tensor_su2::fuse(b1,b2), tensor_su2.cpp:839-909).
 Fused make_fused(const Leg& block, bool left);
 
+// A converged lower state for excited-state targeting (orth[j] of
+// dmrg_su2, read by read_mps, dmrg_su2_omp.cpp:944-1068).  Each site k is
+// held in both forms: left[k] maps fuse(beta[k-1], site) to beta[k] and
+// right[k] maps fuse(site, beta[k]) to beta[k-1] (the reference re-labels
+// orth between the two with left2right/right2left_vectran; for spin-1/2
+// sites that is a per-block transpose with factor 1).  beta[k] is the bond
+// right of site k (beta[-1] = {0}, beta[ns-1] = {2S}).  OL[i] / OR[i] are the
+// overlap environments with this run (ovlp[j][i], dmrg_su2_omp.cpp:327-328,
+// 386, 470-471, 528): legs (beta[i], lb[i]) and (beta[i-1], rb[i]).
+struct LowerState {
+  std::map<int, Leg> beta;
+  std::vector<std::unique_ptr<Tensor>> left, right;
+  double energy = 0;   // Ritz value of the lower run (gs_enr[j])
+  std::map<int, std::unique_ptr<Tensor>> OL, OR;
+};
+
 struct DmrgStats {
   double t_env = 0, t_lanczos = 0, t_svd = 0, t_project = 0, t_start = 0;
   double heff_flops = 0;       // algorithmic FLOPs of all matvecs
@@ -81,6 +97,13 @@ class Dmrg {
   // run's), environments rebuilt from both edges; sweep() continues from the
   // centre.  Returns the saved energy (read_enr, 1100-1152).
   double load(const std::string& dir, int file_dcut, int ky, int exci);
+  // Excited-state targeting (exci > 0): adds the state saved in dir as
+  // uu/enr-{ns}-{file_dcut}-{2S}-{ky}-{exci}-* (file_dcut 0: this run's
+  // max_dcut) to the states every two-site solve is kept orthogonal to, by
+  // the penalty -E_j <phi_j|psi> phi_j of dmrg_su2.cpp:611-623.  Call before
+  // warmup(), or after load() and before the next sweep.
+  void add_lower_state(const std::string& dir, int file_dcut, int ky, int exci);
+  int lower_states() const { return (int)lower_.size(); }
   void set_max_dcut(int d) { D_ = d; }
   // 1: one line per solve on stderr (step, sites, energy, kept multiplets,
   // discarded weight, matvecs, seconds), as the reference logs each step
@@ -116,6 +139,13 @@ class Dmrg {
   std::unique_ptr<Tensor> start_right(int nl, int nr, const Tensor& B, const Leg& b);
   void ensure_left(int i);
   void ensure_right(int i);
+  // overlap environments of every lower state at site i from U_[i] / V_[i]
+  void overlap_left(int i);
+  void overlap_right(int i);
+  void rebuild_overlaps();
+  // each lower state in the two-site basis of (il, il + 1), psi's layout
+  std::vector<std::unique_ptr<Tensor>> lower_vectors(int il, int ir, const Fused& FL,
+                                                     const Fused& FR, const Tensor& psi);
 
   Context& c_;
   Lattice lat_;
@@ -123,6 +153,7 @@ class Dmrg {
   std::map<int, Leg> lb_, rb_;
   std::map<int, std::unique_ptr<Env>> L_, R_;
   std::map<int, std::unique_ptr<Tensor>> U_, V_;   // projectors (U at the centre may carry s)
+  std::vector<std::unique_ptr<LowerState>> lower_;
   std::vector<double> energies_;
   std::vector<StepRecord> history_;
   std::vector<double> wout_;                        // last solve's singular values

@@ -26,13 +26,13 @@
 #include "common.hpp"
 #include "context.hpp"
 #include "dmrg.hpp"
+#include "mpsio.hpp"
 #include "ops.hpp"
 
 namespace su2 {
 
-namespace {
+namespace mpsio {
 
-constexpr int kPhysSpin = 1;
 
 std::string uu_base(const std::string& dir, int ns, int d, int s, int ky, int exci) {
   char b[64];
@@ -45,13 +45,6 @@ std::string tag(int ns, int d, int s, int ky, int exci) {
   std::snprintf(b, sizeof b, "%d-%d-%d-%d-%d", ns, d, s, ky, exci);
   return b;
 }
-
-// one three-leg file: legs, flags and bodies
-struct File3 {
-  Leg leg[3];
-  std::vector<int> flag;
-  std::vector<double> red, cg;
-};
 
 long long block_count(const Leg* l) { return (long long)l[0].nsec() * l[1].nsec() * l[2].nsec(); }
 
@@ -135,6 +128,26 @@ double cg_ratio(const double* stored, const std::vector<double>& ref, const std:
               "load: %s carries a CG that is not a multiple of make_cgc", what.c_str());
   return r;
 }
+
+double read_energy(const std::string& dir, const std::string& t) {
+  std::ifstream f(dir + "/enr-" + t + ".dat");
+  SU2_CHECK(f.is_open(), kErrArgument, "load: no enr-%s.dat in %s", t.c_str(), dir.c_str());
+  double v = 0, energy = 0;
+  bool any = false;
+  while (f >> v) {
+    energy = v;
+    any = true;
+  }
+  SU2_CHECK(any, kErrStructure, "load: empty enr-%s.dat in %s", t.c_str(), dir.c_str());
+  return energy + 1000.0;
+}
+
+}  // namespace mpsio
+
+namespace {
+
+using namespace mpsio;
+constexpr int kPhysSpin = 1;
 
 std::vector<double> download(Context& c, const Tensor& t) {
   std::vector<double> h(t.size());
@@ -272,19 +285,7 @@ double Dmrg::load(const std::string& dir, int file_dcut, int ky, int exci) {
   // read everything first: a missing or malformed file leaves the run intact
   std::vector<File3> files(ns);
   for (int i = 0; i < ns; ++i) files[i] = read_file(base + std::to_string(i));
-  double energy = 0;
-  {
-    std::ifstream f(dir + "/enr-" + tag(ns, fd, S_, ky, exci) + ".dat");
-    SU2_CHECK(f.is_open(), kErrArgument, "load: no enr file in %s", dir.c_str());
-    double v = 0;
-    bool any = false;
-    while (f >> v) {
-      energy = v;
-      any = true;
-    }
-    SU2_CHECK(any, kErrStructure, "load: empty enr file in %s", dir.c_str());
-    energy += 1000.0;
-  }
+  const double energy = read_energy(dir, tag(ns, fd, S_, ky, exci));
   // decode and check every projector against the chain before touching the run
   std::map<int, Leg> lb, rb;
   lb[-1] = make_leg({0}, {1}, 1);
@@ -336,6 +337,11 @@ double Dmrg::load(const std::string& dir, int file_dcut, int ky, int exci) {
   energies_.clear();
   history_.clear();
   wout_.clear();
+  for (auto& st : lower_) {
+    st->OL.erase(st->OL.upper_bound(-1), st->OL.end());
+    st->OR.erase(st->OR.begin(), st->OR.lower_bound(ns));
+  }
+  rebuild_overlaps();
   return energy;
 }
 

@@ -70,6 +70,7 @@ _SIG = {
     "su2_dmrg_set_verbose": (_i, [_p, _i]),
     "su2_dmrg_save": (_i, [_p, _p, C.c_char_p, _i, _i]),
     "su2_dmrg_load": (_i, [_p, _p, C.c_char_p, _i, _i, _i, _dp]),
+    "su2_dmrg_add_lower_state": (_i, [_p, _p, C.c_char_p, _i, _i, _i]),
     "su2_dmrg_stats": (_i, [_p, _dp, _i]),
 }
 
@@ -432,6 +433,12 @@ class Dmrg:
         self.ctx.check(lib.su2_dmrg_load(self.ctx.h, self.h, os.fsencode(directory),
                                          int(file_dcut), ky, exci, C.byref(e)))
         return e.value
+
+    def add_lower_state(self, directory, file_dcut=0, ky=0, exci=0):
+        """Target an excited state: keep the run orthogonal to the state saved
+        in `directory` with index `exci` (call for 0 .. n-1 before warmup)."""
+        self.ctx.check(lib.su2_dmrg_add_lower_state(self.ctx.h, self.h, os.fsencode(directory),
+                                                    int(file_dcut), ky, exci))
 
     def set_verbose(self, level=1):
         self.ctx.check(lib.su2_dmrg_set_verbose(self.h, int(level)))
