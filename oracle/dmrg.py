@@ -34,6 +34,7 @@ import numpy as np
 from scipy.linalg import eigh_tridiagonal
 
 from . import heff
+from . import plaquette
 from . import recoupling as rc
 from .su2 import Bond, dsort2, fuse_bonds
 
@@ -51,9 +52,8 @@ SITE = Bond([PHYS], [1], 1)
 # --------------------------------------------------------------------------
 class Lattice:
     def __init__(self, lx, ly, jcoup, qcoup=0.0, bdry=2):
-        if abs(qcoup) > 1e-5:
-            raise NotImplementedError("plaquette (Q) terms are not part of this oracle")
         self.lx, self.ly, self.ns, self.bdry = lx, ly, lx * ly, bdry
+        self.qcoup = qcoup
         ns = self.ns
         self.hmap = np.zeros((ns, ns), dtype=int)
 
@@ -108,7 +108,17 @@ class Lattice:
                 self.hmap[j, i] = v
         # coupling constants (dmrg_su2.cpp:258-260)
         self.coup = [0.0, jcoup + qcoup / 2.0, 1.0 - abs(qcoup), jcoup + qcoup / 4.0]
-        # retention flags (dmrg_su2.cpp:284-303)
+        # plaquettes (x, y), (x, y+1), (x+1, y+1), (x+1, y) for x < lx-1 as
+        # sorted snake positions (dmrg_su2.cpp:200-232); only with |Q| > 1e-5
+        # (plqflg, dmrg_su2.cpp:263-281)
+        self.plq = []
+        if abs(qcoup) > 1e-5:
+            for m in range(ns - ly):
+                x1, y1 = m // ly, m % ly
+                self.plq.append(tuple(sorted((site(x1, y1), site(x1, (y1 + 1) % ly),
+                                              site(x1 + 1, (y1 + 1) % ly), site(x1 + 1, y1)))))
+        # retention flags (dmrg_su2.cpp:284-303): S_j is kept while a bond
+        # crosses i, S_p0 until p1 and S_p3 until p2 of a plaquette
         self.fll = np.zeros((ns, ns), dtype=int)
         self.frr = np.zeros((ns, ns), dtype=int)
         for i in range(ns):
@@ -118,6 +128,11 @@ class Lattice:
             for j in range(i + 1, ns):
                 if np.any(self.hmap[j, :i] != 0):
                     self.frr[i, j] = 1
+            for p0, p1, p2, p3 in self.plq:
+                if p0 < i < p1:
+                    self.fll[i, p0] = 1
+                if p2 < i < p3:
+                    self.frr[i, p3] = 1
 
     def J(self, i, j):
         return self.coup[self.hmap[i, j]] if self.hmap[i, j] else 0.0
@@ -553,7 +568,7 @@ class DMRG:
     lb[i]: bond right of site i (left part, lb[-1] = {0:1}); rb[i]: bond left
     of site i (right part, rb[ns] = {2S:1}, the auxiliary spin of
     dmrg_su2_omp.cpp:38-42).  U[i]: E_i -> lb[i]; V[i]: F_i -> rb[i].
-    Lenv[i]/Renv[i]: (H, {k: S_k}) in lb[i]/rb[i].
+    Lenv[i]/Renv[i]: (H, {k: S_k}, {(plaquette, channel): {K: op}}) in lb[i]/rb[i].
     """
 
     def __init__(self, lat: Lattice, totspin=0, max_dcut=200, verbose=False, lower=()):
@@ -564,8 +579,8 @@ class DMRG:
         self.verbose = verbose
         self.lb = {-1: Bond([0], [1], 1)}
         self.rb = {self.ns: Bond([totspin], [1], 1)}
-        self.Lenv = {-1: (None, {})}
-        self.Renv = {self.ns: (None, {})}
+        self.Lenv = {-1: (None, {}, {})}
+        self.Renv = {self.ns: (None, {}, {})}
         self.U, self.V = {}, {}
         self.energies = []
         self.history = []          # per step: (kept multiplets, discarded weight, matvecs)
@@ -581,31 +596,39 @@ class DMRG:
     # ---- environments (dmrg_su2_omp.cpp:319-825) --------------------------
     def enlarged_left(self, i):
         E = Enlarged(self.lb[i - 1], left=True)
-        H, S = self.Lenv[i - 1]
+        H, S, P = self.Lenv[i - 1]
         keep = {k: S[k] for k in S if self.lat.fll[i, k]}
         pair = [(self.lat.J(i, k), S[k]) for k in range(i) if self.lat.hmap[i, k] and k in S]
         Hh, Sh, Sn = enlarge(E, H, keep, pair, left=True)
         Sh[i] = Sn
-        return E, Hh, Sh
+        Ph, hadd = plaquette.grow(self.lat, E, i, P, S, True, SIGMA_LEFT, SIGMA_RIGHT)
+        if hadd:
+            for (j, _), blk in hadd.items():
+                Hh[j] = Hh[j] + blk
+        return E, Hh, Sh, Ph
 
     def enlarged_right(self, i):
         E = Enlarged(self.rb[i + 1], left=False)
-        H, S = self.Renv[i + 1]
+        H, S, P = self.Renv[i + 1]
         keep = {k: S[k] for k in S if self.lat.frr[i, k]}
         pair = [(self.lat.J(i, k), S[k]) for k in range(i + 1, self.ns)
                 if self.lat.hmap[i, k] and k in S]
         Hh, Sh, Sn = enlarge(E, H, keep, pair, left=False)
         Sh[i] = Sn
-        return E, Hh, Sh
+        Ph, hadd = plaquette.grow(self.lat, E, i, P, S, False, SIGMA_RIGHT, SIGMA_LEFT)
+        if hadd:
+            for (j, _), blk in hadd.items():
+                Hh[j] = Hh[j] + blk
+        return E, Hh, Sh, Ph
 
     def push_left(self, i, U=None):
         """Truncated left environment after site i (prepare_site_operator_from_left(i,flag))."""
-        E, Hh, Sh = self.enlarged_left(i)
+        E, Hh, Sh, Ph = self.enlarged_left(i)
         if U is None:
             U = _ident(E.dims)
         self.U[i] = U
         self.lb[i] = _bond_of(U)
-        self.Lenv[i] = project(U, Hh, Sh)
+        self.Lenv[i] = project(U, Hh, Sh) + (plaquette.project(U, Ph),)
         for j, st in enumerate(self.lower):
             # overlap_transformation (tensor_su2_dmrg_src.cpp:531-688): the
             # lower state's site tensor against this run's, through ovlp[i-1]
@@ -626,12 +649,12 @@ class DMRG:
             self.OL[j][i] = out
 
     def push_right(self, i, V=None):
-        E, Hh, Sh = self.enlarged_right(i)
+        E, Hh, Sh, Ph = self.enlarged_right(i)
         if V is None:
             V = _ident(E.dims)
         self.V[i] = V
         self.rb[i] = _bond_of(V)
-        self.Renv[i] = project(V, Hh, Sh)
+        self.Renv[i] = project(V, Hh, Sh) + (plaquette.project(V, Ph),)
         for j, st in enumerate(self.lower):
             Fj, Pj = st.right_view(i)
             prev = self.OR[j][i + 1]
@@ -672,8 +695,8 @@ class DMRG:
         return out
 
     def step(self, il, ir, psi0, p1, p2):
-        El, Hl, Sl = self.enlarged_left(il)
-        Er, Hr, Sr = self.enlarged_right(ir)
+        El, Hl, Sl, Pl = self.enlarged_left(il)
+        Er, Hr, Sr, Pr = self.enlarged_right(ir)
         lay = {j: (El.dims[j], Er.dims[j]) for j in El.dims if j in Er.dims}
         if psi0 is None:
             psi0 = {j: np.ones(d) for j, d in lay.items()}
@@ -685,6 +708,7 @@ class DMRG:
             # that output on the operators' full legs).
             psi0 = {j: psi0[j] if j in psi0 else np.zeros(d) for j, d in lay.items()}
         terms = self.terms(il, ir, Sl, Sr)
+        terms += plaquette.heff_terms(self.lat, il, ir, Sl, Sr, Pl, Pr)
         self.last_terms = (Hl, Hr, terms)
 
         phis = self.lower_vectors(il, ir, El, Er, lay) if il + 1 == ir else []

@@ -109,3 +109,59 @@ def sector_energies(lat, twoS=0, count=2, k=16):
     if len(out) < count:
         raise RuntimeError("not enough levels with the requested total spin among the lowest")
     return out
+
+
+def jq_model(lx, ly, j1, j2, q):
+    """Bonds and plaquettes of the J1-J2-Q model on an lx (open) x ly
+    (periodic) cylinder, from coordinates alone (site x*ly + y):
+    H = j1 sum_nn S.S + j2 sum_diag S.S
+        - q sum_plaquettes (P_ij P_kl + P_il P_jk),  P = 1/4 - S.S,
+    with plaquette (i, j, k, l) = (x,y), (x,y+1), (x+1,y+1), (x+1,y) for
+    x < lx-1 and every y (ly >= 3 so that no bond or plaquette repeats)."""
+    if ly < 3:
+        raise ValueError("jq_model needs ly >= 3")
+
+    def s(x, y):
+        return x * ly + (y % ly)
+
+    nn, diag, plq = set(), set(), []
+    for x in range(lx):
+        for y in range(ly):
+            nn.add(tuple(sorted((s(x, y), s(x, y + 1)))))
+            if x + 1 < lx:
+                nn.add(tuple(sorted((s(x, y), s(x + 1, y)))))
+                diag.add(tuple(sorted((s(x, y), s(x + 1, y + 1)))))
+                diag.add(tuple(sorted((s(x + 1, y), s(x, y + 1)))))
+                plq.append((s(x, y), s(x, y + 1), s(x + 1, y + 1), s(x + 1, y)))
+    bonds = [(i, j, j1) for i, j in sorted(nn)] + [(i, j, j2) for i, j in sorted(diag)]
+    return bonds, plq
+
+
+def jq_energies(lx, ly, j1, j2, q, twoS=0, count=1, k=8):
+    """Lowest `count` energies of total spin S = twoS/2 of jq_model."""
+    ns = lx * ly
+    states = _basis(ns, twoS)
+    bonds, plq = jq_model(lx, ly, j1, j2, q)
+    H = _heisenberg(ns, bonds, states)
+    n = len(states)
+    one = sp.identity(n, format="csr")
+
+    def proj(i, j):
+        return 0.25 * one - _heisenberg(ns, [(i, j, 1.0)], states)
+
+    for (i, j, kk, l) in plq:
+        H = H - q * (proj(i, j) @ proj(kk, l) + proj(i, l) @ proj(j, kk))
+    if n <= 600:
+        w, v = np.linalg.eigh(H.toarray())
+    else:
+        w, v = spla.eigsh(H, k=min(k, n - 2), which="SA", tol=1e-13)
+        order = np.argsort(w)
+        w, v = w[order], v[:, order]
+    target = 0.25 * twoS * (twoS + 2)
+    out = []
+    for e, vec in zip(w, v.T):
+        if abs(_s2(ns, states, vec) - target) < 1e-6:
+            out.append(float(e))
+            if len(out) == count:
+                return out
+    raise RuntimeError("no level with the requested total spin among the lowest")

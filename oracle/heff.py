@@ -47,7 +47,7 @@ def presum(ops):
     return out
 
 
-def term_apply(psi, opl, opr):
+def term_apply(psi, opl, opr, K=OP_TWOJ):
     """tensor_su2::hamiltonian_vector_multiplication (tensor_su2_dmrg_src.cpp:893-1041).
 
     For every left block (j -> j') and wavefunction block j, step1 = psi^T L
@@ -63,7 +63,7 @@ def term_apply(psi, opl, opr):
         R = opr.get((ja, jap))
         if R is None:
             continue
-        fac = fac_hamilt_vec(ja, OP_TWOJ, jap)
+        fac = fac_hamilt_vec(ja, K, jap)
         if abs(fac) < FAC_SKIP:
             continue
         tmp = (step1.T @ R) * fac                   # [D_a(j'), D_b(j')]
@@ -79,7 +79,9 @@ def heff_apply(psi, hl, hr, terms):
 
     ``hl``/``hr`` are the block Hamiltonians (or None), ``terms`` a list of
     (left op, right op) pairs whose right (or left) side has already been
-    summed with its couplings (``presum``).  Returns psi' with the same blocks
+    summed with its couplings (``presum``), or (left, right, K) triples whose
+    operator leg is 2K instead of 2 (plaquette terms; K = 0 is the
+    reference's two-leg product L^T psi R, dmrg_su2.cpp:902-906).  Returns psi' with the same blocks
     as ``psi``, which must list every sector of the two-site space (zeros where
     the reference holds a null block): the reference's output lives on the
     operators' full legs (tensor_su2_dmrg_src.cpp:921-928) and its operator+=
@@ -96,8 +98,8 @@ def heff_apply(psi, hl, hr, terms):
             if j in hr:
                 out[j] = out[j] + b @ hr[j]
     acc = {}
-    for opl, opr in terms:                          # dmrg_su2.cpp:582-610
-        t = term_apply(psi, opl, opr)
+    for term in terms:                              # dmrg_su2.cpp:582-610, 635-655
+        t = term_apply(psi, *term)
         for j, b in t.items():
             acc[j] = acc[j] + b if j in acc else b
     for j, b in acc.items():                        # dmrg_su2.cpp:657-660

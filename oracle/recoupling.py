@@ -154,3 +154,66 @@ def right_to_left_psi(jl, s, jr):
     src = np.einsum("lxe,er->lxr", cgc(jl, s, jr), singlet(jr))
     dst = np.einsum("lf,xrf->lxr", singlet(jl), cgc(s, jr, jl))
     return float(np.sum(src * dst) / np.sum(dst * dst))
+
+
+# --------------------------------------------------------------------------
+# plaquette (Q) terms: two spin-1 operators coupled to rank K, and back
+# --------------------------------------------------------------------------
+# The reference carries the nested plaquette product (S_p0.S_p3)(S_p1.S_p2)
+# between p1 and p2 as one operator whose operator leg holds the ranks 0, 1
+# and 2 of S_p0 (x) s_p1 (permutation with the qterm tensors,
+# tensor_su2_hamiltonian.cpp:19-76, 871-1108; dmrg_su2_omp.cpp:411-447).
+# Here each rank K (2K = 0, 2, 4) is an operator of its own:
+#   X^K_Q = sum_qp C(2,2,K)[q,p,Q] A_q s_p          (couple)
+#   Z_q   = sum_K z_K sum_Qp C(K,2,2)[Q,p,q] X^K_Q s'_p   (reduce to rank 1)
+# with A_q[out,in] = A[in,q,out] the block operator and s the site spin.
+# REDUCE_WEIGHT holds z_K such that Z = A (s.s') in the normalisation of the
+# environment operators, and the H_eff contraction of X^K with its right
+# counterpart Y^K (fac_hamilt_vec) sums to (A.D)(s.s') with weight 1 per K;
+# both are fixed by dense four-spin operator identities in
+# tests/test_oracle_plaquette.py.
+REDUCE_WEIGHT = {0: -1.0 / math.sqrt(3.0), 2: 1.0, 4: -math.sqrt(5.0 / 3.0)}
+
+
+@lru_cache(maxsize=None)
+def couple_left(jl, ja, jlp, jap, K, s=1):
+    """Block vector operator (x) site spin coupled to rank K, left enlarged leg:
+    out[a,Q,a'] = sum C(jl,s,ja)[l,x,a] C(jl,2,jlp)[l,q,m] C(s,2,s)[x,p,y]
+                      C(2,2,K)[q,p,Q] C(jlp,s,jap)[m,y,b]."""
+    if not (_tri(jl, s, ja) and _tri(jlp, s, jap) and _tri(jl, 2, jlp) and _tri(ja, K, jap)):
+        return 0.0
+    t = np.einsum("lxa,lqm,xpy,qpQ,myb->aQb", cgc(jl, s, ja), cgc(jl, 2, jlp), cgc(s, 2, s),
+                  cgc(2, 2, K), cgc(jlp, s, jap))
+    return _ratio(t, cgc(ja, K, jap))
+
+
+@lru_cache(maxsize=None)
+def couple_right(jr, ja, jrp, jap, K, s=1):
+    """Right enlarged leg (fusion C(s,jr,ja)) version of couple_left."""
+    if not (_tri(s, jr, ja) and _tri(s, jrp, jap) and _tri(jr, 2, jrp) and _tri(ja, K, jap)):
+        return 0.0
+    t = np.einsum("xra,rqm,xpy,qpQ,ymb->aQb", cgc(s, jr, ja), cgc(jr, 2, jrp), cgc(s, 2, s),
+                  cgc(2, 2, K), cgc(s, jrp, jap))
+    return _ratio(t, cgc(ja, K, jap))
+
+
+@lru_cache(maxsize=None)
+def reduce_left(jl, ja, jlp, jap, K, s=1):
+    """Block rank-K operator (x) site spin coupled to rank 1, left enlarged leg:
+    out[a,q,a'] = sum C(jl,s,ja)[l,x,a] C(jl,K,jlp)[l,Q,m] C(s,2,s)[x,p,y]
+                      C(K,2,2)[Q,p,q] C(jlp,s,jap)[m,y,b]."""
+    if not (_tri(jl, s, ja) and _tri(jlp, s, jap) and _tri(jl, K, jlp) and _tri(ja, 2, jap)):
+        return 0.0
+    t = np.einsum("lxa,lQm,xpy,Qpq,myb->aqb", cgc(jl, s, ja), cgc(jl, K, jlp), cgc(s, 2, s),
+                  cgc(K, 2, 2), cgc(jlp, s, jap))
+    return _ratio(t, cgc(ja, 2, jap))
+
+
+@lru_cache(maxsize=None)
+def reduce_right(jr, ja, jrp, jap, K, s=1):
+    """Right enlarged leg version of reduce_left."""
+    if not (_tri(s, jr, ja) and _tri(s, jrp, jap) and _tri(jr, K, jrp) and _tri(ja, 2, jap)):
+        return 0.0
+    t = np.einsum("xra,rQm,xpy,Qpq,ymb->aqb", cgc(s, jr, ja), cgc(jr, K, jrp), cgc(s, 2, s),
+                  cgc(K, 2, 2), cgc(s, jrp, jap))
+    return _ratio(t, cgc(ja, 2, jap))
