@@ -91,7 +91,9 @@ def sweep_leg(L, ctx, lx, ly, dcut, variants):
         "matvecs": int(d["matvecs"]),
         "matvec_tflops": round(d["heff_flops"] / max(d["t_matvec"], 1e-12) / 1e12, 3),
         "breakdown_s": {k: round(d[k], 3) for k in
-                        ("t_env", "t_lanczos", "t_matvec", "t_svd", "t_project", "t_start")},
+                        ("t_env", "t_lanczos", "t_matvec", "t_svd", "t_jacobi", "t_project",
+                         "t_start")},
+        "jacobi_sweeps_per_step": round(d["jacobi_sweeps"] / max(nstep, 1), 2),
     }
     dm.close()
     return out
@@ -111,6 +113,10 @@ def main():
     ap.add_argument("--sweep-lattice", default="16x8",
                     help="lattice of the timed sweep, or 'none' to skip the sweep leg")
     ap.add_argument("--sweep-d", type=int, default=2000, help="max multiplets of the sweep leg")
+    ap.add_argument("--svd-block", type=int, default=32, choices=[16, 32],
+                    help="column block width of the Jacobi SVD")
+    ap.add_argument("--svd-inner", type=int, default=1, choices=[1, 2, 3],
+                    help="inner Jacobi sweeps per visit of a column pair")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -124,6 +130,7 @@ def main():
     work = getattr(synthetic, args.workload)(seed=0)
     flops = synthetic.flops(work)
     ctx = L.Context(0)
+    ctx.set_svd_tuning(args.svd_block, args.svd_inner)
     dev = L.upload_workload(ctx, work)
     plan = dev["plan"]
     alg, executed = plan.flops()
@@ -197,6 +204,8 @@ def main():
             "sweep_time_s": None if sweep is None else sweep.get("sweep_time_s"),
             "sweep": sweep,
             "gemm_variants": list(chosen),
+            "svd_block_cols": args.svd_block,
+            "svd_inner_sweeps": args.svd_inner,
         },
         "roofline": {
             "bound": "mfma",

@@ -55,6 +55,11 @@ int su2_ctx_create(int device, su2_ctx** out);
 int su2_ctx_destroy(su2_ctx* ctx);
 const char* su2_last_error(const su2_ctx* ctx);
 int su2_ctx_synchronize(su2_ctx* ctx);
+/* Tuning of the one-sided Jacobi SVD behind su2_svd_truncate and the DMRG
+ * solves: column block width 16 or 32 (default 32), inner Jacobi sweeps per
+ * visit of a pair 1..3 (default 1).  No counterpart in the reference; the
+ * results agree to the SVD's own accuracy either way. */
+int su2_ctx_set_svd_tuning(su2_ctx* ctx, int block_cols, int inner_sweeps);
 
 /* Block-sparse tensor on the device: su2struct + reduced blocks
  * (tensor_su2::set_tensor_su2(su2struct&, double*, double*, int*),
@@ -147,7 +152,12 @@ int su2_svd_truncate(su2_ctx* ctx, const su2_tensor* psi, double p1, double p2, 
  *   kind 5 fac_operator_pairup_rght          labels (ja, jr, jr', k, s0, s1)
  *   kind 6/7 wavefunction re-coupling across one site (jl, s, jr), left->right
  *          and right->left (no reference table: used for start vectors)
- *   kind 8 fac_hamilt_vec                    labels (j, k, j') */
+ *   kind 8 fac_hamilt_vec                    labels (j, k, j')
+ *   kind 9/10  plaquette coupling of a block vector operator and the site
+ *          spin to rank K, left / right   labels (jl, ja, jl', ja', 2K, s)
+ *   kind 11/12 plaquette reduction of a block rank-K operator and the site
+ *          spin to rank 1, left / right   labels (jl, ja, jl', ja', 2K, s)
+ *          (the reference's fac_permutation_* role, dmrg_su2.cpp:947-1018) */
 int su2_recoupling(int kind, const int* labels, double* out);
 
 /* Two-site SU(2) DMRG on an lx x ly cylinder (open x, periodic y) for total
@@ -212,9 +222,10 @@ int su2_dmrg_energies(const su2_dmrg* d, double* out, int cap, int* n);
  * (lanczos_su2.cpp:237-338).  Any output may be NULL; *n receives the count. */
 int su2_dmrg_history(const su2_dmrg* d, int* kept, double* discarded, int* matvecs, int cap,
                      int* n);
-/* Timing counters: out[0..9] = environment s, Lanczos s, SVD s, projection s,
- * start-vector s, device matvec s, algorithmic H_eff FLOPs, matvecs, steps,
- * last centre bond dimension. */
+/* Timing counters: out[0..11] = environment s, Lanczos s, SVD s, projection
+ * s, start-vector s, device matvec s, algorithmic H_eff FLOPs, matvecs, steps,
+ * last centre bond dimension, s inside the one-sided Jacobi SVD, its sweeps
+ * summed over solves.  cap bounds what is written. */
 int su2_dmrg_stats(const su2_dmrg* d, double* out, int cap);
 
 #ifdef __cplusplus

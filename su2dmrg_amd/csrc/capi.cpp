@@ -94,6 +94,18 @@ int su2_ctx_synchronize(su2_ctx* ctx) {
   });
 }
 
+int su2_ctx_set_svd_tuning(su2_ctx* ctx, int block_cols, int inner_sweeps) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    SU2_CHECK(block_cols == 16 || block_cols == 32, su2::kErrArgument,
+              "svd block width %d (16 or 32)", block_cols);
+    SU2_CHECK(inner_sweeps >= 1 && inner_sweeps <= 3, su2::kErrArgument,
+              "svd inner sweeps %d (1..3)", inner_sweeps);
+    ctx->c.svd_block_cols = block_cols;
+    ctx->c.svd_inner_sweeps = inner_sweeps;
+  });
+}
+
 int su2_tensor_create(su2_ctx* ctx, int nleg, const int* nsec, const int* twoj, const int* dims,
                       const int* dirs, int locspin, const int* present, su2_tensor** out) {
   return guard(ctx, [&] {
@@ -424,6 +436,10 @@ int su2_recoupling(int kind, const int* labels, double* out) {
       case 6: *out = su2::rc_psi_left_to_right(l[0], l[1], l[2]); break;
       case 7: *out = su2::rc_psi_right_to_left(l[0], l[1], l[2]); break;
       case 8: *out = su2::heff_recoupling(l[0], l[1], l[2]); break;
+      case 9: *out = su2::rc_couple_left(l[0], l[1], l[2], l[3], l[4], l[5]); break;
+      case 10: *out = su2::rc_couple_right(l[0], l[1], l[2], l[3], l[4], l[5]); break;
+      case 11: *out = su2::rc_reduce_left(l[0], l[1], l[2], l[3], l[4], l[5]); break;
+      case 12: *out = su2::rc_reduce_right(l[0], l[1], l[2], l[3], l[4], l[5]); break;
       default: su2::fail(su2::kErrArgument, "recoupling kind %d", kind);
     }
   });
@@ -545,11 +561,12 @@ int su2_dmrg_stats(const su2_dmrg* d, double* out, int cap) {
     need(d, "dmrg");
     need(out, "out");
     const su2::DmrgStats& s = d->d.stats();
-    const double v[10] = {s.t_env,      s.t_lanczos,         s.t_svd,
+    const double v[12] = {s.t_env,      s.t_lanczos,         s.t_svd,
                           s.t_project,  s.t_start,           s.t_matvec,
                           s.heff_flops, (double)s.matvecs,   (double)s.steps,
-                          (double)d->d.last_bond_dim()};
-    for (int i = 0; i < 10 && i < cap; ++i) out[i] = v[i];
+                          (double)d->d.last_bond_dim(), s.t_jacobi,
+                          (double)s.jacobi_sweeps};
+    for (int i = 0; i < 12 && i < cap; ++i) out[i] = v[i];
   });
 }
 

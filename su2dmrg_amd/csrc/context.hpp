@@ -33,6 +33,8 @@ class Context {
   double* host_scratch(size_t n);
 
   int max_dcut = 8000;  // reference default (main_omp.cpp:15)
+  int svd_block_cols = 32;   // one-sided Jacobi column block width (16 or 32)
+  int svd_inner_sweeps = 1;  // its inner sweeps per visit of a pair (1..3)
 
  private:
   int device_;

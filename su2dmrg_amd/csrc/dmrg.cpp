@@ -217,6 +217,7 @@ std::unique_ptr<Env> Dmrg::enlarge(const Fused& E, const Env& env, const std::ve
     }
   }
   sb.run(c);
+  grow_plaquettes(E, env, newsite, left, *out);
   return out;
 }
 
@@ -256,6 +257,14 @@ std::unique_ptr<Env> Dmrg::project(const Tensor& P, const Env& hat) {
     out->S[kv.first] = vector_op(c, out->bond);
     ops.push_back({kv.second.get(), out->S[kv.first].get()});
   }
+  for (auto& kv : hat.P)
+    for (auto& r : kv.second) {
+      auto& dst = out->P[kv.first][r.first];
+      dst.reset(new Tensor(&c, {with_dir(out->bond, 1), make_leg({r.first}, {1}, 1),
+                                with_dir(out->bond, -1)},
+                           0, nullptr));
+      ops.push_back({r.second.get(), dst.get()});
+    }
   // Y = O[a->b] P[b], then O'[a->b] = P[a]^T Y
   struct Job {
     const double* O;
@@ -273,7 +282,7 @@ std::unique_ptr<Env> Dmrg::project(const Tensor& P, const Env& hat) {
     const int last = T.nleg() - 1;
     for (auto& tb : T.blocks()) {
       const int a = tb.twoj[0], b = tb.twoj[last];
-      const int ob = last == 1 ? O.find2(a, b) : O.find3(a, kOp, b);
+      const int ob = last == 1 ? O.find2(a, b) : O.find3(a, O.leg(1).twoj[0], b);
       const int pa = P.find2(a, a), pbk = P.find2(b, b);
       SU2_CHECK(ob >= 0 && pa >= 0 && pbk >= 0, kErrStructure, "project: missing block");
       Job j;
@@ -357,6 +366,7 @@ Dmrg::StepOut Dmrg::step(int il, int ir, std::unique_ptr<Tensor> psi0, double p1
       terms.push_back(t);
     }
   }
+  plaquette_terms(il, ir, El, Er, terms);
   HeffPlan plan(c, *psi, El.H.get(), Er.H.get(), terms);
   if (var1_ >= 0) plan.set_variants(var1_, var2_);
   const int mlanc = (int)std::min<long long>(kMlanc, psi->size() + 1);
@@ -392,6 +402,8 @@ Dmrg::StepOut Dmrg::step(int il, int ir, std::unique_ptr<Tensor> psi0, double p1
   t0 = now();
   SvdResult sv = truncated_svd(c, *psi, p1, p2, D_);
   stats_.t_svd += now() - t0;
+  stats_.t_jacobi += sv.t_jacobi;
+  stats_.jacobi_sweeps += sv.jacobi_sweeps;
   o.energy = lr.energy;
   o.U = std::move(sv.U);
   o.V = std::move(sv.V);

@@ -30,11 +30,16 @@
 namespace su2 {
 
 class Context;
+struct HeffTerm;
 
 struct Env {
   Leg bond;
   std::unique_ptr<Tensor> H;                      // scalar block Hamiltonian (may be null)
   std::map<int, std::unique_ptr<Tensor>> S;       // site -> spin operator
+  // partial plaquette operators plqopr (dmrg_su2_omp.cpp:411-447, 553-590):
+  // (plaquette, channel) -> operator rank 2K -> three-leg operator whose
+  // operator leg is the single multiplet 2K
+  std::map<std::pair<int, int>, std::map<int, std::unique_ptr<Tensor>>> P;
 };
 
 struct Fused {
@@ -80,6 +85,8 @@ struct DmrgStats {
   long long matvecs = 0;
   long long steps = 0;
   double t_matvec = 0;         // device time inside H_eff applies (events)
+  double t_jacobi = 0;         // time inside the one-sided Jacobi SVD
+  long long jacobi_sweeps = 0; // its sweeps, summed over solves
 };
 
 class Dmrg {
@@ -133,6 +140,12 @@ class Dmrg {
   std::unique_ptr<Env> enlarge(const Fused& E, const Env& env, const std::vector<int>& keep,
                                const std::vector<std::pair<double, int>>& pair, int newsite,
                                bool left);
+  // the plaquette terms' partial operators of a block after adding site i
+  // (out.H receives the plaquettes completed at i)
+  void grow_plaquettes(const Fused& E, const Env& env, int i, bool left, Env& out);
+  // H_eff terms of the plaquettes that the cut (il, il + 1) splits
+  void plaquette_terms(int il, int ir, const Env& El, const Env& Er,
+                       std::vector<HeffTerm>& terms) const;
   std::unique_ptr<Env> project(const Tensor& P, const Env& hat);
   StepOut step(int il, int ir, std::unique_ptr<Tensor> psi0, double p1, double p2);
   std::unique_ptr<Tensor> start_left(int nl, int nr, const Tensor& A, const Leg& b);

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <tuple>
 
 #include "blas1.hpp"
 #include "cg.hpp"
@@ -13,18 +14,17 @@ namespace su2 {
 
 namespace {
 
-constexpr int kOpTwoj = 2;            // spin-1 operator leg
 constexpr double kFacSkip = 1e-12;    // tensor_su2_dmrg_src.cpp:996
 constexpr int kTargetUnits = 2048;    // ~8 work units per CU on 256 CUs
 
-void check_op(const Tensor& op, const Leg& in, const Leg& out, const char* what) {
+void check_op(const Tensor& op, const Leg& in, const Leg& out, int k, const char* what) {
   SU2_CHECK(op.nleg() == 3, kErrStructure, "%s operator must have 3 legs", what);
   SU2_CHECK(op.leg(0).same_sectors(in), kErrStructure,
             "%s operator leg 0 does not match the wavefunction leg", what);
   SU2_CHECK(op.leg(2).same_sectors(out), kErrStructure,
             "%s operator leg 2 does not match the wavefunction leg", what);
-  SU2_CHECK(op.leg(1).nsec() == 1 && op.leg(1).twoj[0] == kOpTwoj && op.leg(1).dim[0] == 1,
-            kErrStructure, "%s operator leg 1 must be a single spin-1 multiplet", what);
+  SU2_CHECK(op.leg(1).nsec() == 1 && op.leg(1).twoj[0] == k && op.leg(1).dim[0] == 1,
+            kErrStructure, "%s operator leg 1 must be a single multiplet 2j=%d", what, k);
 }
 
 void check_scalar(const Tensor& h, const Leg& l, const char* what) {
@@ -48,8 +48,10 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
   const int T = (int)terms.size();
   for (auto& t : terms) {
     SU2_CHECK(!t.left.empty() && !t.right.empty(), kErrArgument, "H_eff term with an empty side");
-    for (auto& p : t.left) check_op(*p.second, la, la, "left");
-    for (auto& p : t.right) check_op(*p.second, lb, lb, "right");
+    SU2_CHECK(t.k >= 0 && t.k % 2 == 0 && t.k <= 8, kErrArgument, "H_eff term operator leg %d",
+              t.k);
+    for (auto& p : t.left) check_op(*p.second, la, la, t.k, "left");
+    for (auto& p : t.right) check_op(*p.second, lb, lb, t.k, "right");
   }
   psi_size_ = psi.size();
 
@@ -57,33 +59,44 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
   std::map<int, const Block*> pb;
   for (auto& b : psi.blocks()) pb[b.twoj[0]] = &b;
 
-  // transitions (j -> j') present on both sides of at least one term
+  // terms by operator leg: a transition (j -> j', k) bundles every term of
+  // leg k, at its position in that list
+  std::map<int, std::vector<int>> by_k;
+  std::vector<int> local(T, -1);
+  for (int ti = 0; ti < T; ++ti) {
+    local[ti] = (int)by_k[terms[ti].k].size();
+    by_k[terms[ti].k].push_back(ti);
+  }
+
+  // transitions (j -> j', k) present on both sides of at least one term
   struct Tr {
-    int ja, jap;
+    int ja, jap, k, nt;
     double fac;
     int Da, Dap, Db, Dbp;
     long long lb_off, rb_off, x_off;
   };
   std::vector<Tr> trs;
   {
-    std::map<std::pair<int, int>, int> seen;
+    std::map<std::tuple<int, int, int>, int> seen;
     for (auto& t : terms)
       for (auto& p : t.left)
         for (auto& blk : p.second->blocks()) {
           const int ja = blk.twoj[0], jap = blk.twoj[2];
           if (!pb.count(ja) || !pb.count(jap)) continue;
-          if (seen.count({ja, jap})) continue;
-          const double f = heff_recoupling(ja, kOpTwoj, jap);
+          if (seen.count({ja, jap, t.k})) continue;
+          const double f = heff_recoupling(ja, t.k, jap);
           if (std::fabs(f) < kFacSkip) continue;
           bool right_has = false;
-          for (auto& tt : terms)
-            for (auto& q : tt.right)
-              if (q.second->find3(ja, kOpTwoj, jap) >= 0) right_has = true;
+          for (int tj : by_k[t.k])
+            for (auto& q : terms[tj].right)
+              if (q.second->find3(ja, t.k, jap) >= 0) right_has = true;
           if (!right_has) continue;
-          seen[{ja, jap}] = 1;
+          seen[{ja, jap, t.k}] = 1;
           Tr tr{};
           tr.ja = ja;
           tr.jap = jap;
+          tr.k = t.k;
+          tr.nt = (int)by_k[t.k].size();
           tr.fac = f;
           tr.Da = la.dim_of(ja);
           tr.Dap = la.dim_of(jap);
@@ -92,7 +105,9 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
           trs.push_back(tr);
         }
     std::sort(trs.begin(), trs.end(), [](const Tr& x, const Tr& y) {
-      return x.jap != y.jap ? x.jap < y.jap : x.ja < y.ja;
+      if (x.jap != y.jap) return x.jap < y.jap;
+      if (x.ja != y.ja) return x.ja < y.ja;
+      return x.k < y.k;
     });
   }
 
@@ -100,9 +115,10 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
   // on both sides), plus the block-Hamiltonian products
   for (auto& t : terms) {
     for (auto& tr : trs) {
+      if (tr.k != t.k) continue;
       bool l = false, r = false;
-      for (auto& p : t.left) l = l || p.second->find3(tr.ja, kOpTwoj, tr.jap) >= 0;
-      for (auto& p : t.right) r = r || p.second->find3(tr.ja, kOpTwoj, tr.jap) >= 0;
+      for (auto& p : t.left) l = l || p.second->find3(tr.ja, t.k, tr.jap) >= 0;
+      for (auto& p : t.right) r = r || p.second->find3(tr.ja, t.k, tr.jap) >= 0;
       if (l && r)
         alg_flops_ += 2.0 * tr.Db * tr.Da * tr.Dap + 2.0 * tr.Dap * tr.Db * tr.Dbp;
     }
@@ -117,11 +133,11 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
   long long work = 0;
   for (auto& tr : trs) {
     tr.lb_off = work;
-    work += (long long)tr.Da * tr.Dap * T;
+    work += (long long)tr.Da * tr.Dap * tr.nt;
     tr.rb_off = work;
-    work += (long long)T * tr.Db * tr.Dbp;
+    work += (long long)tr.nt * tr.Db * tr.Dbp;
     tr.x_off = work;
-    work += (long long)tr.Dap * T * tr.Db;
+    work += (long long)tr.Dap * tr.nt * tr.Db;
   }
   SU2_HIP(hipMalloc(&d_in_, std::max<long long>(psi_size_, 1) * sizeof(double)));
   SU2_HIP(hipMalloc(&d_out_, std::max<long long>(psi_size_, 1) * sizeof(double)));
@@ -137,13 +153,15 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
     long long total = 0;
     for (int ti = 0; ti < T; ++ti) {
       const HeffTerm& t = terms[ti];
+      const int tl = local[ti];
       for (auto& tr : trs) {
+        if (tr.k != t.k) continue;
         if (pos < t.left.size()) {
           const Tensor& op = *t.left[pos].second;
-          int id = op.find3(tr.ja, kOpTwoj, tr.jap);
+          int id = op.find3(tr.ja, t.k, tr.jap);
           if (id >= 0) {
             StridedAxpy d{};
-            d.dst = d_work_ + tr.lb_off + (long long)ti * tr.Da * tr.Dap;
+            d.dst = d_work_ + tr.lb_off + (long long)tl * tr.Da * tr.Dap;
             d.src = op.data() + op.blocks()[id].offset;
             d.rs = 1;
             d.cs = tr.Da;
@@ -158,13 +176,13 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
         }
         if (pos < t.right.size()) {
           const Tensor& op = *t.right[pos].second;
-          int id = op.find3(tr.ja, kOpTwoj, tr.jap);
+          int id = op.find3(tr.ja, t.k, tr.jap);
           if (id >= 0) {
             StridedAxpy d{};
-            d.dst = d_work_ + tr.rb_off + ti;
+            d.dst = d_work_ + tr.rb_off + tl;
             d.src = op.data() + op.blocks()[id].offset;
-            d.rs = T;
-            d.cs = (long long)T * tr.Db;
+            d.rs = tr.nt;
+            d.cs = (long long)tr.nt * tr.Db;
             d.ld = tr.Db;
             d.rows = tr.Db;
             d.cols = tr.Dbp;
@@ -188,9 +206,9 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
 
   // stage 1: X(j->j') = f * Lb^T psi(j)   [T*D_a(j') x D_b(j)]
   for (auto& tr : trs) {
-    if (T == 0) break;
     const Block* bj = pb[tr.ja];
-    int p = stage1_.add_problem(d_work_ + tr.x_off, (long long)T * tr.Dap, T * tr.Dap, tr.Db);
+    int p = stage1_.add_problem(d_work_ + tr.x_off, (long long)tr.nt * tr.Dap, tr.nt * tr.Dap,
+                                tr.Db);
     stage1_.add_slice(p, d_work_ + tr.lb_off, tr.Da, 1, d_in_ + bj->offset, tr.Da, 0, tr.Da,
                       tr.fac);
   }
@@ -205,10 +223,10 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
       any = true;
     };
     for (auto& tr : trs) {
-      if (tr.jap != jap || T == 0) continue;
+      if (tr.jap != jap) continue;
       ensure();
-      stage2_.add_slice(p, d_work_ + tr.x_off, Dap, 0, d_work_ + tr.rb_off, (long long)T * tr.Db,
-                        0, T * tr.Db, 1.0);
+      stage2_.add_slice(p, d_work_ + tr.x_off, Dap, 0, d_work_ + tr.rb_off,
+                        (long long)tr.nt * tr.Db, 0, tr.nt * tr.Db, 1.0);
     }
     if (hl) {
       int id = hl->find2(jap, jap);

@@ -27,8 +27,13 @@
 
 namespace su2 {
 
+// One term sum_a c_a L_a (x) sum_b c_b R_b of H_eff; every operator has the
+// operator leg 2j = k (2: spin operators; 0 and 4: the plaquette terms'
+// rank-0 and rank-2 parts, contracted with fac_hamilt_vec[j][k][j'] like
+// the spin operators, dmrg_su2.cpp:635-655).
 struct HeffTerm {
   std::vector<std::pair<double, const Tensor*>> left, right;
+  int k = 2;
 };
 
 class HeffPlan {

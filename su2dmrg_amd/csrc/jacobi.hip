@@ -14,13 +14,10 @@ namespace su2 {
 
 namespace {
 
-constexpr int CB = 32;          // columns per block
-constexpr int NP = 2 * CB;      // largest pair problem
 constexpr int RCH = 32;         // rows per staged chunk
-constexpr int NT = 256;         // threads per pair workgroup
-constexpr int INNER_SWEEPS = 3; // inner Jacobi sweeps per visit of a pair
+constexpr int INNER_SWEEPS = 3; // most inner Jacobi sweeps per visit of a pair
 constexpr int MAX_SLICES = 8;   // row slices per pair
-constexpr int SLICE_TARGET = 256;   // workgroups a round aims for (one per CU)
+constexpr int MIN_SLICE_ROWS = 64;
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
@@ -51,15 +48,19 @@ __device__ __forceinline__ void slice_rows(int L, int P, int s, int& lo, int& hi
 }
 
 // Partial Gram matrix of one pair over one row slice of W, on the matrix
-// cores.  32-row chunks of the pair's columns are staged in LDS (columns >= n
-// zero); wave w accumulates the 16 x 16 tiles (w, 0..3) with
-// v_mfma_f64_16x16x4f64: A = X^T fragment (lane l: X[r + l/16][16 w + l%16]),
-// B = X fragment (lane l: X[r + l/16][16 j + l%16]), D lane l:
-// G[16 w + l/16 + 4 q][16 j + l%16].  The 64 x 64 result goes row-major to
-// gpart[blockIdx.x] (workgroup = job * pmax + slice).
+// cores.  32-row chunks of the pair's NP = 2 CB columns are staged in LDS
+// (columns >= n zero); the (NP/16)^2 output tiles of 16 x 16 are spread over
+// the waves, each tile a v_mfma_f64_16x16x4f64 chain: A = X^T fragment
+// (lane l: X[r + l/16][16 tr + l%16]), B = X fragment (lane l:
+// X[r + l/16][16 tc + l%16]), D lane l: G[16 tr + l/16 + 4 q][16 tc + l%16].
+// The NP x NP result goes row-major to gpart[blockIdx.x] (workgroup = job *
+// pmax + slice).
+template <int CB, int NT>
 __global__ __launch_bounds__(NT) void gram_partial_kernel(const BlockDesc* __restrict__ blocks,
                                                           const PairJob* __restrict__ jobs,
                                                           int pmax, double* __restrict__ gpart) {
+  constexpr int NP = 2 * CB, TT = NP / 16, NW = NT / 64, TPW = TT * TT / NW;
+  static_assert(TPW * NW == TT * TT, "tiles must split evenly over the waves");
   __shared__ double X[RCH][NP + 1];
   __shared__ int colmap[NP];
   const int job = blockIdx.x / pmax, slice = blockIdx.x - job * pmax;
@@ -71,9 +72,9 @@ __global__ __launch_bounds__(NT) void gram_partial_kernel(const BlockDesc* __res
   for (int c = t; c < n; c += NT) colmap[c] = c < J.bp ? J.p0 + c : J.q0 + (c - J.bp);
   int lo, hi;
   slice_rows(B.L, J.P, slice, lo, hi);
-  f64x4 acc[4];
+  f64x4 acc[TPW];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) acc[j] = f64x4{0.0, 0.0, 0.0, 0.0};
+  for (int j = 0; j < TPW; ++j) acc[j] = f64x4{0.0, 0.0, 0.0, 0.0};
   __syncthreads();
   for (int r0 = lo; r0 < hi; r0 += RCH) {
     const int nr = min(RCH, hi - r0);
@@ -85,19 +86,23 @@ __global__ __launch_bounds__(NT) void gram_partial_kernel(const BlockDesc* __res
 #pragma unroll
     for (int k4 = 0; k4 < RCH; k4 += 4) {
       const int row = k4 + (lane >> 4);
-      const double a = X[row][16 * wave + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, X[row][16 * j + (lane & 15)], acc[j], 0, 0, 0);
+      for (int j = 0; j < TPW; ++j) {
+        const int tau = wave * TPW + j, tr = tau / TT, tc = tau % TT;
+        acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(X[row][16 * tr + (lane & 15)],
+                                                      X[row][16 * tc + (lane & 15)], acc[j], 0, 0, 0);
+      }
     }
     __syncthreads();
   }
   double* out = gpart + (size_t)blockIdx.x * NP * NP;
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
+  for (int j = 0; j < TPW; ++j) {
+    const int tau = wave * TPW + j, tr = tau / TT, tc = tau % TT;
 #pragma unroll
     for (int q = 0; q < 4; ++q)
-      out[(16 * wave + (lane >> 4) + 4 * q) * NP + 16 * j + (lane & 15)] = acc[j][q];
+      out[(16 * tr + (lane >> 4) + 4 * q) * NP + 16 * tc + (lane & 15)] = acc[j][q];
+  }
 }
 
 // Orthogonalise one pair of column blocks: every row slice of the pair sums
@@ -108,24 +113,31 @@ __global__ __launch_bounds__(NT) void gram_partial_kernel(const BlockDesc* __res
 // ordered bits of a non-negative double).
 //
 // The pair's Gram matrix is diagonalised in LDS by parallel cyclic Jacobi
-// (round-robin rounds of disjoint rotations, at most INNER_SWEEPS sweeps: the
-// outer iteration revisits the pair, and more inner sweeps do not lower the
-// outer sweep count).  A round updates G <- J^T G J by 2 x 2 blocks, one
+// (round-robin rounds of disjoint rotations, inner_sweeps <= INNER_SWEEPS
+// sweeps: the outer iteration revisits the pair; one inner sweep per visit
+// costs a few more outer sweeps but the least time, since the round is
+// bound by its VALU instructions).  A round updates G <- J^T G J by 2 x 2 blocks, one
 // thread per block pair, with the rotated entry set to zero and the diagonal
 // moved by t g_pq; a rotation is skipped below max(tol_in sqrt(g_pp g_qq),
 // 0.1 tol_abs ||A||_F (sqrt g_pp + sqrt g_qq)), ten times below the outer
 // test.
+template <int CB, int NT>
 __global__ __launch_bounds__(NT) void jacobi_rotate_kernel(const BlockDesc* __restrict__ blocks,
                                                            const PairJob* __restrict__ jobs,
                                                            int pmax,
                                                            const double* __restrict__ gpart,
                                                            double tol_rel, double tol_abs,
-                                                           unsigned long long* __restrict__ off) {
+                                                           unsigned long long* __restrict__ off,
+                                                           int inner_sweeps) {
+  constexpr int NP = 2 * CB, HALF = NP / 2, TT = NP / 16, NW = NT / 64;
+  constexpr int RT = RCH / 16, TPW = RT * TT / NW;
+  static_assert(TPW * NW == RT * TT, "tiles must split evenly over the waves");
+  static_assert(NT >= HALF && NT % NP == 0, "thread mapping");
   __shared__ double G[NP][NP + 1];
   __shared__ double R[NP][NP + 1];
   __shared__ double X[RCH][NP + 1];
-  __shared__ double cs[NP / 2], sn[NP / 2], tn[NP / 2];
-  __shared__ int pp[NP / 2], qq[NP / 2];
+  __shared__ double cs[HALF], sn[HALF], tn[HALF];
+  __shared__ int pp[HALF], qq[HALF];
   __shared__ int colmap[NP];
   __shared__ double red[NT];
   __shared__ int rotated[INNER_SWEEPS];
@@ -140,13 +152,28 @@ __global__ __launch_bounds__(NT) void jacobi_rotate_kernel(const BlockDesc* __re
   for (int c = t; c < n; c += NT) colmap[c] = c < J.bp ? J.p0 + c : J.q0 + (c - J.bp);
   if (t < INNER_SWEEPS) rotated[t] = 0;
   {
+    // the slices' partial Gram matrices, summed in slice order; all loads of
+    // a thread are issued before the sums
     const double* gp = gpart + (size_t)job * pmax * NP * NP;
-    for (int e = t; e < NP * NP; e += NT) {
-      const int a = e / NP, b = e % NP;
-      double g = 0.0;
-      for (int s = 0; s < J.P; ++s) g += gp[(size_t)s * NP * NP + e];
-      G[a][b] = g;
-      R[a][b] = a == b ? 1.0 : 0.0;
+    constexpr int PER = NP * NP / NT, GRP = 4;
+    static_assert(PER * NT == NP * NP && PER % GRP == 0, "Gram elements per thread");
+    for (int i0 = 0; i0 < PER; i0 += GRP) {
+      double part[GRP][MAX_SLICES];
+#pragma unroll
+      for (int i = 0; i < GRP; ++i)
+#pragma unroll
+        for (int s = 0; s < MAX_SLICES; ++s)
+          part[i][s] = s < J.P ? gp[(size_t)s * NP * NP + t + NT * (i0 + i)] : 0.0;
+#pragma unroll
+      for (int i = 0; i < GRP; ++i) {
+        const int e = t + NT * (i0 + i), a = e / NP, b = e % NP;
+        double g = 0.0;
+#pragma unroll
+        for (int s = 0; s < MAX_SLICES; ++s)
+          if (s < J.P) g += part[i][s];
+        G[a][b] = g;
+        R[a][b] = a == b ? 1.0 : 0.0;
+      }
     }
   }
   __syncthreads();
@@ -191,7 +218,7 @@ __global__ __launch_bounds__(NT) void jacobi_rotate_kernel(const BlockDesc* __re
   const int npair = m / 2;
   const double tol_in = 1e-14;
   const double fl = 0.1 * tol_abs * B.normF;
-  for (int sweep = 0; sweep < INNER_SWEEPS; ++sweep) {
+  for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
     for (int r = 0; r < m - 1; ++r) {
       if (t < npair) {
         const int p = rr_p(r, t, m), q = rr_q(r, t, m);
@@ -218,30 +245,79 @@ __global__ __launch_bounds__(NT) void jacobi_rotate_kernel(const BlockDesc* __re
       }
       __syncthreads();
       {
-        // G <- J^T G J on the 2 x 2 blocks (k, k2), k <= k2, mirrored to (k2, k)
-        const int k = t & 31;
-        if (k < npair) {
-          const int p = pp[k], q = qq[k];
-          const double c = cs[k], s = sn[k];
-          for (int k2 = t >> 5; k2 < npair; k2 += NT / 32) {
-            if (k2 < k) continue;
-            if (k2 == k) {
-              if (s == 0.0) continue;
-              const double apq = G[p][q];
-              G[p][p] -= tn[k] * apq;
-              G[q][q] += tn[k] * apq;
-              G[p][q] = 0.0;
-              G[q][p] = 0.0;
-              continue;
-            }
-            const double c2 = cs[k2], s2 = sn[k2];
-            if (s == 0.0 && s2 == 0.0) continue;
-            const int p2 = pp[k2], q2 = qq[k2];
-            const double m00 = G[p][p2], m01 = G[p][q2], m10 = G[q][p2], m11 = G[q][q2];
-            const double t00 = c2 * m00 - s2 * m01, t01 = s2 * m00 + c2 * m01;
-            const double t10 = c2 * m10 - s2 * m11, t11 = s2 * m10 + c2 * m11;
+        // G <- J^T G J on the 2 x 2 blocks (k, k2), k <= k2, mirrored to (k2,
+        // k), and R <- R J.  Every element belongs to one thread in a round,
+        // so each thread first loads all its blocks and R entries, then
+        // computes, then stores: one LDS latency per phase, not per block.
+        constexpr int KS = NT / HALF, MB = (HALF + KS - 1) / KS;
+        constexpr int RS = NT / NP, MR = (HALF + RS - 1) / RS;
+        const int k = t % HALF, kb = t / HALF;
+        const bool kin = k < npair;
+        const int p = kin ? pp[k] : 0, q = kin ? qq[k] : 0;
+        const double c = kin ? cs[k] : 1.0, s = kin ? sn[k] : 0.0, tk = kin ? tn[k] : 0.0;
+        double g[MB][4], c2v[MB], s2v[MB];
+        int p2v[MB], q2v[MB];
+        int kind[MB];   // 0 nothing, 1 diagonal block, 2 off-diagonal block
+#pragma unroll
+        for (int i = 0; i < MB; ++i) {
+          const int k2 = kb + KS * i;
+          kind[i] = 0;
+          p2v[i] = 0;
+          q2v[i] = 0;
+          c2v[i] = 1.0;
+          s2v[i] = 0.0;
+          if (!kin || k2 >= npair || k2 < k) continue;
+          if (k2 == k) {
+            if (s == 0.0) continue;
+            kind[i] = 1;
+            g[i][0] = G[p][q];
+            g[i][1] = G[p][p];
+            g[i][2] = G[q][q];
+            continue;
+          }
+          const double s2 = sn[k2];
+          if (s == 0.0 && s2 == 0.0) continue;
+          kind[i] = 2;
+          s2v[i] = s2;
+          c2v[i] = cs[k2];
+          p2v[i] = pp[k2];
+          q2v[i] = qq[k2];
+          g[i][0] = G[p][p2v[i]];
+          g[i][1] = G[p][q2v[i]];
+          g[i][2] = G[q][p2v[i]];
+          g[i][3] = G[q][q2v[i]];
+        }
+        const int row = t % NP, rb = t / NP;
+        double ru[MR], rv[MR], rc[MR], rsn[MR];
+        int rp[MR], rq[MR];
+#pragma unroll
+        for (int i = 0; i < MR; ++i) {
+          const int kk = rb + RS * i;
+          rsn[i] = kk < npair ? sn[kk] : 0.0;
+          rc[i] = 1.0;
+          rp[i] = 0;
+          rq[i] = 0;
+          if (rsn[i] == 0.0) continue;
+          rc[i] = cs[kk];
+          rp[i] = pp[kk];
+          rq[i] = qq[kk];
+          ru[i] = R[row][rp[i]];
+          rv[i] = R[row][rq[i]];
+        }
+#pragma unroll
+        for (int i = 0; i < MB; ++i) {
+          if (kind[i] == 1) {
+            G[p][p] = g[i][1] - tk * g[i][0];
+            G[q][q] = g[i][2] + tk * g[i][0];
+            G[p][q] = 0.0;
+            G[q][p] = 0.0;
+          } else if (kind[i] == 2) {
+            const double c2 = c2v[i], s2 = s2v[i];
+            const double t00 = c2 * g[i][0] - s2 * g[i][1], t01 = s2 * g[i][0] + c2 * g[i][1];
+            const double t10 = c2 * g[i][2] - s2 * g[i][3], t11 = s2 * g[i][2] + c2 * g[i][3];
             const double n00 = c * t00 - s * t10, n01 = c * t01 - s * t11;
             const double n10 = s * t00 + c * t10, n11 = s * t01 + c * t11;
+            const int p2 = p2v[i], q2 = q2v[i];
             G[p][p2] = n00;
             G[p][q2] = n01;
             G[q][p2] = n10;
@@ -252,16 +328,12 @@ __global__ __launch_bounds__(NT) void jacobi_rotate_kernel(const BlockDesc* __re
             G[q2][q] = n11;
           }
         }
-        // R <- R J (rows >= n hold zeros in the rotated columns)
-        const int row = t & (NP - 1);
-        for (int kk = t / NP; kk < npair; kk += NT / NP) {
-          const double s = sn[kk];
-          if (s == 0.0) continue;
-          const double c = cs[kk];
-          const int p = pp[kk], q = qq[kk];
-          const double u = R[row][p], v = R[row][q];
-          R[row][p] = c * u - s * v;
-          R[row][q] = s * u + c * v;
+        // (rows >= n hold zeros in the rotated columns)
+#pragma unroll
+        for (int i = 0; i < MR; ++i) {
+          if (rsn[i] == 0.0) continue;
+          R[row][rp[i]] = rc[i] * ru[i] - rsn[i] * rv[i];
+          R[row][rq[i]] = rsn[i] * ru[i] + rc[i] * rv[i];
         }
       }
       __syncthreads();
@@ -270,12 +342,11 @@ __global__ __launch_bounds__(NT) void jacobi_rotate_kernel(const BlockDesc* __re
   }
 
   // ---- rotate this slice's rows: W[:, cols] <- W[:, cols] R, same for V ----
-  // Y = X R per 32-row chunk on the matrix cores: wave w owns the row tile
-  // w % 2 and the column tiles 2 (w / 2) and 2 (w / 2) + 1; A = X fragment
-  // (lane l: X[16 rt + l%16][k + l/16]), B = R fragment (lane l: R[k + l/16][16 ct + l%16]).
+  // Y = X R per 32-row chunk on the matrix cores, the (RCH/16) x (NP/16)
+  // output tiles spread over the waves: A = X fragment (lane l:
+  // X[16 rt + l%16][k + l/16]), B = R fragment (lane l: R[k + l/16][16 ct + l%16]).
   // The result goes back through LDS so each column segment is written whole.
   const int lane = t & 63, wave = t >> 6;
-  const int rt = wave & 1, ct0 = 2 * (wave >> 1);
   for (int which = 0; which < 2; ++which) {
     double* M = which == 0 ? B.W : B.V;
     const long long ld = which == 0 ? B.L : B.k;
@@ -289,21 +360,27 @@ __global__ __launch_bounds__(NT) void jacobi_rotate_kernel(const BlockDesc* __re
         X[rr][c] = (rr < nr && c < n) ? M[(r0 + rr) + (long long)colmap[c] * ld] : 0.0;
       }
       __syncthreads();
-      f64x4 y[2] = {f64x4{0.0, 0.0, 0.0, 0.0}, f64x4{0.0, 0.0, 0.0, 0.0}};
+      f64x4 y[TPW];
+#pragma unroll
+      for (int u = 0; u < TPW; ++u) y[u] = f64x4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int k = 0; k < NP; k += 4) {
-        const double a = X[16 * rt + (lane & 15)][k + (lane >> 4)];
 #pragma unroll
-        for (int u = 0; u < 2; ++u)
-          y[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, R[k + (lane >> 4)][16 * (ct0 + u) + (lane & 15)],
+        for (int u = 0; u < TPW; ++u) {
+          const int tau = wave * TPW + u, rt = tau % RT, ct = tau / RT;
+          y[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(X[16 * rt + (lane & 15)][k + (lane >> 4)],
+                                                      R[k + (lane >> 4)][16 * ct + (lane & 15)],
                                                       y[u], 0, 0, 0);
+        }
       }
       __syncthreads();
 #pragma unroll
-      for (int u = 0; u < 2; ++u)
+      for (int u = 0; u < TPW; ++u) {
+        const int tau = wave * TPW + u, rt = tau % RT, ct = tau / RT;
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-          X[16 * rt + (lane >> 4) + 4 * q][16 * (ct0 + u) + (lane & 15)] = y[u][q];
+          X[16 * rt + (lane >> 4) + 4 * q][16 * ct + (lane & 15)] = y[u][q];
+      }
       __syncthreads();
       for (int e = t; e < RCH * n; e += NT) {
         const int rr = e % RCH, c = e / RCH;
@@ -417,10 +494,19 @@ struct DevMem {
 
 }  // namespace
 
-void jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol, int max_sweeps) {
+int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol, int max_sweeps,
+                       int block_cols, int inner_sweeps) {
   hipStream_t st = c.stream();
   const int nb = (int)blocks.size();
-  if (nb == 0) return;
+  if (nb == 0) return 0;
+  SU2_CHECK(block_cols == 16 || block_cols == 32, kErrArgument, "jacobi: block width %d",
+            block_cols);
+  SU2_CHECK(inner_sweeps >= 1 && inner_sweeps <= INNER_SWEEPS, kErrArgument,
+            "jacobi: inner sweeps %d", inner_sweeps);
+  // pairs of 16-column blocks run one wave per workgroup (several per CU);
+  // pairs of 32-column blocks run four waves
+  const int CB = block_cols, NP = 2 * CB;
+  const int slice_target = CB == 16 ? 1024 : 256;
   std::vector<int> L(nb), K(nb);
   std::vector<long long> noff(nb + 1, 0);
   for (int b = 0; b < nb; ++b) {
@@ -499,10 +585,11 @@ void jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol
   for (size_t r = 0; r < rounds.size(); ++r) {
     round_off[r] = (int)all.size();
     const int nj = (int)rounds[r].size();
-    const int pmax = nj == 0 ? 1 : std::min(MAX_SLICES, std::max(1, (SLICE_TARGET + nj - 1) / nj));
+    // at most slice_target workgroups: one residency wave of the rotate kernel
+    const int pmax = nj == 0 ? 1 : std::min(MAX_SLICES, std::max(1, slice_target / nj));
     int used = 1;
     for (PairJob& j : rounds[r]) {
-      j.P = std::max(1, std::min(pmax, L[j.blk] / 64));
+      j.P = std::max(1, std::min(pmax, L[j.blk] / MIN_SLICE_ROWS));
       used = std::max(used, j.P);
     }
     round_pmax[r] = used;
@@ -519,7 +606,9 @@ void jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol
   std::vector<unsigned long long> hoff(nb);
   bool converged = all.empty();
   double worst = 0;
+  int sweeps_done = 0;
   for (int sweep = 0; sweep < max_sweeps && !converged; ++sweep) {
+    ++sweeps_done;
     // descending column norms
     if (sweep > 0) norms();
     for (int b = 0; b < nb; ++b) {
@@ -550,11 +639,19 @@ void jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol
       if (nj == 0) continue;
       const int pm = round_pmax[r];
       const PairJob* jr = (const PairJob*)djobs.p + round_off[r];
-      hipLaunchKernelGGL(gram_partial_kernel, dim3(nj * pm), dim3(NT), 0, st,
-                         (const BlockDesc*)ddesc.p, jr, pm, (double*)dgpart.p);
-      hipLaunchKernelGGL(jacobi_rotate_kernel, dim3(nj * pm), dim3(NT), 0, st,
-                         (const BlockDesc*)ddesc.p, jr, pm, (const double*)dgpart.p, tol, 1e-14,
-                         (unsigned long long*)doff.p);
+      if (CB == 16) {
+        hipLaunchKernelGGL((gram_partial_kernel<16, 64>), dim3(nj * pm), dim3(64), 0, st,
+                           (const BlockDesc*)ddesc.p, jr, pm, (double*)dgpart.p);
+        hipLaunchKernelGGL((jacobi_rotate_kernel<16, 64>), dim3(nj * pm), dim3(64), 0, st,
+                           (const BlockDesc*)ddesc.p, jr, pm, (const double*)dgpart.p, tol, 1e-14,
+                           (unsigned long long*)doff.p, inner_sweeps);
+      } else {
+        hipLaunchKernelGGL((gram_partial_kernel<32, 256>), dim3(nj * pm), dim3(256), 0, st,
+                           (const BlockDesc*)ddesc.p, jr, pm, (double*)dgpart.p);
+        hipLaunchKernelGGL((jacobi_rotate_kernel<32, 256>), dim3(nj * pm), dim3(256), 0, st,
+                           (const BlockDesc*)ddesc.p, jr, pm, (const double*)dgpart.p, tol, 1e-14,
+                           (unsigned long long*)doff.p, inner_sweeps);
+      }
     }
     SU2_HIP(hipGetLastError());
     SU2_HIP(hipMemcpyAsync(hoff.data(), doff.p, sizeof(unsigned long long) * nb,
@@ -615,6 +712,7 @@ void jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol
     SU2_HIP(hipStreamSynchronize(st));
   }
   SU2_HIP(hipGetLastError());
+  return sweeps_done;
 }
 
 }  // namespace su2

@@ -10,12 +10,12 @@
 // the singular values, accurate relative to each value rather than to the
 // largest one.
 //
-// Columns are cut into blocks of 32.  A sweep visits every pair of blocks
+// Columns are cut into blocks of 16 (or 32).  A sweep visits every pair of blocks
 // once, in round-robin rounds of disjoint pairs; two kernel launches run all
 // pairs of a round for every block of the batch.  The first forms partial
-// Gram matrices (up to 64 x 64) of each pair over row slices of W, the second
+// Gram matrices (up to 32 x 32) of each pair over row slices of W, the second
 // sums them, diagonalises the pair's Gram matrix in LDS by parallel cyclic
-// Jacobi (at most three inner sweeps per visit) and applies the eigenvector
+// Jacobi (one to three inner sweeps per visit) and applies the eigenvector
 // matrix to the pair's columns of W and V, each slice to its own rows.  A
 // round with few pairs cuts every pair into up to 8 slices so that it spans
 // the chip.  Before each sweep the columns are put in descending norm order.
@@ -23,7 +23,8 @@
 // |g_ij| <= max(tol * sqrt(g_ii g_jj), 1e-14 ||A||_F (sqrt g_ii + sqrt g_jj))
 // (the absolute floor ends the iteration on columns that carry rounding
 // errors of order eps ||A||); columns with norm <= 1e-13 ||A||_F are left
-// out of the test.  Blocks of up to 32 columns are one self-pair per sweep.
+// out of the test.  A block that fits one column block is one self-pair per
+// sweep.
 #pragma once
 
 #include <vector>
@@ -41,11 +42,14 @@ struct JacobiBlock {
   std::vector<double> s;   // out: k singular values, ascending
 };
 
-// Decomposes every block; throws kErrNumeric when max_sweeps pass without
-// convergence.  W and V columns are sorted ascending by singular value; each
+// Decomposes every block and returns the number of sweeps it took; throws
+// kErrNumeric when max_sweeps pass without convergence.  W and V columns are sorted ascending by singular value; each
 // column of V has its first entry with |v| >= max|v|/2 made positive (the
 // LAPACK routine the reference calls leaves signs open), W's columns follow.
-void jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol = 1e-13,
-                        int max_sweeps = 40);
+// block_cols: 16 (pairs of 32 columns, one wave per pair) or 32 (pairs of 64
+// columns, four waves per pair); inner_sweeps: 1..3 inner Jacobi sweeps per
+// visit of a pair.
+int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol = 1e-13,
+                       int max_sweeps = 40, int block_cols = 32, int inner_sweeps = 1);
 
 }  // namespace su2

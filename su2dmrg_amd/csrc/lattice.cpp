@@ -1,5 +1,6 @@
 #include "lattice.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 #include "common.hpp"
@@ -11,8 +12,8 @@ Lattice::Lattice(int lx_, int ly_, double jc, double qc, int bd)
   SU2_CHECK(lx >= 1 && ly >= 1 && ns >= 4, kErrArgument, "lattice %dx%d too small", lx, ly);
   SU2_CHECK(bdry == 0 || bdry == 1 || bdry == 2, kErrArgument, "boundary code %d not supported",
             bdry);
-  SU2_CHECK(std::fabs(qcoup) <= 1e-5, kErrArgument,
-            "plaquette (Q) terms are not supported (qcoup=%g)", qcoup);
+  SU2_CHECK(std::fabs(qcoup) <= 1e-5 || ly >= 2, kErrArgument,
+            "plaquette (Q) terms need ly >= 2 (qcoup=%g)", qcoup);
   hmap.assign((size_t)ns * ns, 0);
   auto site = [&](int x, int y) { return x * ly + ((x % 2 == 0) ? y : ly - 1 - y); };
   for (int k = 0; k <= 4; ++k) {
@@ -52,6 +53,15 @@ Lattice::Lattice(int lx_, int ly_, double jc, double qc, int bd)
   coup[1] = jcoup + qcoup / 2.0;
   coup[2] = 1.0 - std::fabs(qcoup);
   coup[3] = jcoup + qcoup / 4.0;
+  if (std::fabs(qcoup) > 1e-5) {
+    for (int m = 0; m < ns - ly; ++m) {
+      const int x1 = m / ly, y1 = m % ly;
+      std::array<int, 4> p = {site(x1, y1), site(x1, (y1 + 1) % ly),
+                              site(x1 + 1, (y1 + 1) % ly), site(x1 + 1, y1)};
+      std::sort(p.begin(), p.end());
+      plq.push_back(p);
+    }
+  }
   fll.assign((size_t)ns * ns, 0);
   frr.assign((size_t)ns * ns, 0);
   for (int i = 0; i < ns; ++i) {
@@ -61,6 +71,11 @@ Lattice::Lattice(int lx_, int ly_, double jc, double qc, int bd)
     for (int j = i + 1; j < ns; ++j)
       for (int k = 0; k < i; ++k)
         if (h(j, k)) frr[(size_t)i * ns + j] = 1;
+    // S_p0 stays until p1 and S_p3 until p2 of a plaquette (dmrg_su2.cpp:293-302)
+    for (auto& p : plq) {
+      if (p[0] < i && i < p[1]) fll[(size_t)i * ns + p[0]] = 1;
+      if (p[2] < i && i < p[3]) frr[(size_t)i * ns + p[3]] = 1;
+    }
   }
 }
 

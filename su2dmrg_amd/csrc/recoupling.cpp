@@ -215,6 +215,88 @@ double rc_pairup_right(int ja, int jr, int jrp, int k, int s0, int s1) {
 
 namespace {
 
+// Block operator (rank kb, CG C(j,kb,jp)) times the site spin (C(s,2,s)),
+// their operator legs coupled by C(kb,2,ko), in the enlarged basis:
+//   out[a,qo,b] = sum F(l,x,a) C(j,kb,jp)[l,qb,m] C(s,2,s)[x,p,y]
+//                     C(kb,2,ko)[qb,p,qo] F'(m,y,b)
+// with F = C(j,s,ja)[l,x,a] on the left and C(s,j,ja)[x,l,a] on the right.
+double coupled_site(bool right, int j, int ja, int jp, int jap, int kb, int ko, int s) {
+  T3 f1 = right ? cg(s, j, ja) : cg(j, s, ja);
+  T3 f2 = right ? cg(s, jp, jap) : cg(jp, s, jap);
+  T3 bo = cg(j, kb, jp), st = cg(s, 2, s), cc = cg(kb, 2, ko);
+  // W[x,y,qb,qo] = sum_p C(s,2,s)[x,p,y] C(kb,2,ko)[qb,p,qo]
+  const int ns = s + 1, nb = kb + 1, no = ko + 1;
+  std::vector<double> W((size_t)ns * ns * nb * no, 0.0);
+  for (int x = 0; x < ns; ++x)
+    for (int y = 0; y < ns; ++y)
+      for (int qb = 0; qb < nb; ++qb)
+        for (int qo = 0; qo < no; ++qo) {
+          double acc = 0;
+          for (int p = 0; p <= 2; ++p) acc += st(x, p, y) * cc(qb, p, qo);
+          W[x + ns * (y + ns * (qb + (size_t)nb * qo))] = acc;
+        }
+  auto F1 = [&](int l, int x, int a) { return right ? f1(x, l, a) : f1(l, x, a); };
+  auto F2 = [&](int m, int y, int b) { return right ? f2(y, m, b) : f2(m, y, b); };
+  T3 out(ja + 1, ko + 1, jap + 1);
+  for (int a = 0; a <= ja; ++a)
+    for (int qo = 0; qo < no; ++qo)
+      for (int b = 0; b <= jap; ++b) {
+        double acc = 0;
+        for (int l = 0; l <= j; ++l)
+          for (int x = 0; x < ns; ++x) {
+            const double u = F1(l, x, a);
+            if (u == 0) continue;
+            for (int m = 0; m <= jp; ++m)
+              for (int y = 0; y < ns; ++y) {
+                const double v = F2(m, y, b);
+                if (v == 0) continue;
+                for (int qb = 0; qb < nb; ++qb)
+                  acc += u * bo(l, qb, m) * W[x + ns * (y + ns * (qb + (size_t)nb * qo))] * v;
+              }
+          }
+        out(a, qo, b) = acc;
+      }
+  return ratio(out.v, cg_tensor(ja, ko, jap));
+}
+
+}  // namespace
+
+double rc_couple_left(int jl, int ja, int jlp, int jap, int K, int s) {
+  if (!(tri(jl, s, ja) && tri(jlp, s, jap) && tri(jl, 2, jlp) && tri(ja, K, jap))) return 0.0;
+  return cached({9, jl, ja, jlp, jap, K, s},
+                [&] { return coupled_site(false, jl, ja, jlp, jap, 2, K, s); });
+}
+
+double rc_couple_right(int jr, int ja, int jrp, int jap, int K, int s) {
+  if (!(tri(s, jr, ja) && tri(s, jrp, jap) && tri(jr, 2, jrp) && tri(ja, K, jap))) return 0.0;
+  return cached({10, jr, ja, jrp, jap, K, s},
+                [&] { return coupled_site(true, jr, ja, jrp, jap, 2, K, s); });
+}
+
+double rc_reduce_left(int jl, int ja, int jlp, int jap, int K, int s) {
+  if (!(tri(jl, s, ja) && tri(jlp, s, jap) && tri(jl, K, jlp) && tri(ja, 2, jap))) return 0.0;
+  return cached({11, jl, ja, jlp, jap, K, s},
+                [&] { return coupled_site(false, jl, ja, jlp, jap, K, 2, s); });
+}
+
+double rc_reduce_right(int jr, int ja, int jrp, int jap, int K, int s) {
+  if (!(tri(s, jr, ja) && tri(s, jrp, jap) && tri(jr, K, jrp) && tri(ja, 2, jap))) return 0.0;
+  return cached({12, jr, ja, jrp, jap, K, s},
+                [&] { return coupled_site(true, jr, ja, jrp, jap, K, 2, s); });
+}
+
+double rc_reduce_weight(int K) {
+  switch (K) {
+    case 0: return -1.0 / std::sqrt(3.0);
+    case 2: return 1.0;
+    case 4: return -std::sqrt(5.0 / 3.0);
+    default: fail(kErrArgument, "reduce weight: operator rank 2K=%d", K);
+  }
+  return 0.0;
+}
+
+namespace {
+
 // src: singlet between l and F=(s r)->l;  dst: (l s)->E=r then singlet to r
 void psi_pair(int jl, int s, int jr, std::vector<double>& a, std::vector<double>& b) {
   T3 cf = cg(s, jr, jl), ce = cg(jl, s, jr);

@@ -1,6 +1,7 @@
 #include "svd.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 
 #include "blas1.hpp"
@@ -67,6 +68,8 @@ SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int
   // (A V for m >= n, A^T U for m < n: the reference's u = A v), "V" the short
   // side's eigenvectors, both in ascending order of the singular values.
   std::vector<std::vector<double>> wv(secs.size());
+  int jacobi_sweeps = 0;
+  double t_jacobi = 0;
   {
     std::vector<JacobiBlock> jb;
     std::vector<size_t> idx;
@@ -85,7 +88,9 @@ SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int
       jb.push_back(b);
       idx.push_back(i);
     }
-    jacobi_svd_batched(c, jb);
+    const auto tj = std::chrono::steady_clock::now();
+    jacobi_sweeps = jacobi_svd_batched(c, jb, 1e-13, 40, c.svd_block_cols, c.svd_inner_sweeps);
+    t_jacobi = std::chrono::duration<double>(std::chrono::steady_clock::now() - tj).count();
     for (size_t t = 0; t < jb.size(); ++t) {
       std::vector<double>& w = wv[idx[t]];
       w.resize(jb[t].s.size());
@@ -113,21 +118,20 @@ SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int
     dc = std::min(dc, max_dcut);
     dc = std::min(dc, s.m >= s.n ? s.m : s.n);
     s.dc = dc;
-    // factors for the long-side matrix (held in U) and the eigenvectors (V).
-    // A kept value at or below 1e-11 of the sector's largest still counts as a
-    // multiplet (the reference's rule), but its columns are zeroed: their
-    // directions are rounding noise, and normalising noise (the s^-1 factor
-    // of an isometric side) would put near-duplicate states into the basis.
-    // Such a term carries weight s^2 <= 1e-22 of the sector's.
+    // factors for the long-side matrix (held in U) and the eigenvectors (V),
+    // s^(p-1) and s^p as the reference scales u = A v and v.  A kept value at
+    // or below 1e-11 of the sector's largest still counts as a multiplet (the
+    // reference's rule) and keeps its eigenvector, an orthonormal direction
+    // of the short side as dsyev gives it; only a long-side column that would
+    // be divided by such a value is zeroed, since A v is rounding noise there
+    // and normalising noise would put a near-duplicate state into the basis.
     std::vector<double> fl(k, 0.0), fe(k, 0.0);
     const double pl = s.m >= s.n ? p1 - 1.0 : p2 - 1.0;  // long side: U (m>=n) or V (m<n)
     const double pe = s.m >= s.n ? p2 : p1;
     const double noise = 1e-11 * s.s[k - 1];
     for (int t = k - dc; t < k; ++t) {
-      if (s.s[t] > noise) {
-        fl[t] = std::pow(s.s[t], pl);
-        fe[t] = std::pow(s.s[t], pe);
-      }
+      fe[t] = std::pow(s.s[t], pe);
+      if (s.s[t] > noise || pl >= 0) fl[t] = std::pow(s.s[t], pl);
     }
     dfac = c.reduce_scratch(2 * k);
     std::vector<double> both(fl);
@@ -161,6 +165,8 @@ SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int
     }
   res.bond = make_leg(bt, bd, -1);
   res.discarded_weight = disc;
+  res.jacobi_sweeps = jacobi_sweeps;
+  res.t_jacobi = t_jacobi;
   for (size_t i = 0; i < secs.size(); ++i)
     for (int t = secs[i].k - keep[i]; t < secs[i].k; ++t) res.wout.push_back(secs[i].s[t]);
   for (size_t i = 0; i < secs.size(); ++i)

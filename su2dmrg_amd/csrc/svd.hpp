@@ -29,6 +29,8 @@ struct SvdResult {
   // the reference's wout (tensor_su2.cpp:1707-1731): kept values sector by
   // sector, then the cut ones sector by sector, each ascending
   std::vector<double> wout;
+  int jacobi_sweeps = 0;      // one-sided Jacobi sweeps the sectors took
+  double t_jacobi = 0;        // seconds inside the Jacobi SVD
 };
 
 SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int max_dcut);

@@ -37,6 +37,7 @@ _SIG = {
     "su2_ctx_destroy": (_i, [_p]),
     "su2_last_error": (C.c_char_p, [_p]),
     "su2_ctx_synchronize": (_i, [_p]),
+    "su2_ctx_set_svd_tuning": (_i, [_p, _i, _i]),
     "su2_tensor_create": (_i, [_p, _i, _ip, _ip, _ip, _ip, _i, _ip, C.POINTER(_p)]),
     "su2_tensor_destroy": (_i, [_p, _p]),
     "su2_tensor_size": (_i, [_p, _llp, _ip]),
@@ -114,6 +115,11 @@ class Context:
 
     def sync(self):
         self.check(lib.su2_ctx_synchronize(self.h))
+
+    def set_svd_tuning(self, block_cols=32, inner_sweeps=1):
+        """Column block width (16 or 32) and inner sweeps per pair visit (1..3)
+        of the one-sided Jacobi SVD."""
+        self.check(lib.su2_ctx_set_svd_tuning(self.h, int(block_cols), int(inner_sweeps)))
 
     def close(self):
         if self.h:
@@ -351,7 +357,8 @@ def upload_workload(ctx, work):
 # --------------------------------------------------------------------------
 RC_KINDS = {"op_transform_left": 0, "op_transform_right": 1, "op_initial_left": 2,
             "op_initial_right": 3, "pairup_left": 4, "pairup_right": 5,
-            "psi_left_to_right": 6, "psi_right_to_left": 7, "hamilt_vec": 8}
+            "psi_left_to_right": 6, "psi_right_to_left": 7, "hamilt_vec": 8,
+            "couple_left": 9, "couple_right": 10, "reduce_left": 11, "reduce_right": 12}
 
 
 def recoupling(kind, *labels):
@@ -463,10 +470,10 @@ class Dmrg:
         return [(kept[i], disc[i], mv[i]) for i in range(n.value)]
 
     def stats(self):
-        out = (C.c_double * 10)()
-        self.ctx.check(lib.su2_dmrg_stats(self.h, out, 10))
+        out = (C.c_double * 12)()
+        self.ctx.check(lib.su2_dmrg_stats(self.h, out, 12))
         keys = ["t_env", "t_lanczos", "t_svd", "t_project", "t_start", "t_matvec",
-                "heff_flops", "matvecs", "steps", "bond_dim"]
+                "heff_flops", "matvecs", "steps", "bond_dim", "t_jacobi", "jacobi_sweeps"]
         return dict(zip(keys, list(out)))
 
     def run(self, nsweep=5, stop=1e-6):

@@ -2,8 +2,9 @@
 and exact diagonalisation (GPU).
 
 Step-by-step energy parity is asserted on lattices whose DMRG trajectory is
-well conditioned: perturbing every Gram matrix of the oracle by 1e-15 moves no
-energy by more than 1e-11 (pinned on the CPU in test_oracle_dmrg.py).  On the
+well conditioned: perturbing every Gram matrix of the oracle, and every
+wavefunction before its SVD, by 1e-15 relative moves no energy by more than
+about 1e-11 (pinned on the CPU in test_oracle_dmrg.py).  On the
 periodic cylinders the lattice symmetries make Gram eigenvalues degenerate;
 the warm-up's all-ones start vector (tensor_su2.cpp:2367) in that arbitrary
 basis, fed to a Lanczos run of at most five vectors, then gives
@@ -66,15 +67,17 @@ def test_truncated_svd_matches_oracle(ctx, p1, p2, dcut):
         np.testing.assert_allclose(W.T @ W, np.eye(W.shape[1]), atol=1e-11)
 
 
+@pytest.mark.parametrize("block,inner", [(32, 1), (32, 3), (16, 1)])
 @pytest.mark.parametrize("m,n,decades", [(300, 200, 20), (180, 420, 12), (520, 500, 6)])
-def test_svd_wide_spectrum_against_lapack(ctx, m, n, decades):
+def test_svd_wide_spectrum_against_lapack(ctx, m, n, decades, block, inner):
     # Singular values spread over many decades, as in a converged DMRG
     # wavefunction: every value to 1e-12 of the largest (the Gram route of the
     # reference reaches only eps s_max^2 / s), the reconstruction to the
     # weight of the columns zeroed at s <= 1e-11 s_max, and the left vectors
     # orthogonal to 1e-13 s_max / min(s_i, s_j) where s > 1e-9 s_max.
     from su2dmrg_amd import lib as L
-    _say(f"svd wide spectrum {m}x{n}, {decades} decades")
+    _say(f"svd wide spectrum {m}x{n}, {decades} decades, block {block}, inner {inner}")
+    ctx.set_svd_tuning(block, inner)
     rng = np.random.default_rng(m + n)
     k = min(m, n)
     u, _ = np.linalg.qr(rng.standard_normal((m, k)))
@@ -82,6 +85,7 @@ def test_svd_wide_spectrum_against_lapack(ctx, m, n, decades):
     A = (u * np.logspace(0, -decades, k)) @ v.T
     t = L.wavefunction(ctx, [1], [m], [n], {1: A})
     U, V, kept, disc = L.svd_truncate(ctx, t, 0, 1, 100000)
+    ctx.set_svd_tuning(32, 1)
     ref = np.sort(np.linalg.svd(A, compute_uv=False))
     s = np.array(kept)
     assert len(s) == k

@@ -111,27 +111,66 @@ def test_lanczos_against_eigh():
     assert abs(e2 - np.linalg.eigvalsh(T)[0]) < 1e-12
 
 
-@pytest.mark.parametrize("lx,ly,jcoup,D,twoS", [(4, 2, 1.0, 100, 0), (6, 2, 2.0, 10, 0)])
-def test_open_lattice_trajectory_is_well_conditioned(monkeypatch, lx, ly, jcoup, D, twoS):
-    # The GPU run is compared with the oracle energy by energy on open lattices.
-    # That comparison is meaningful only when the trajectory does not hinge on
-    # roundoff: perturb every Gram matrix by 1e-15 relative, symmetric noise
-    # and require every energy to move by less than 1e-11.
-    lat = od.Lattice(lx, ly, jcoup=jcoup, bdry=0)
+# every step-by-step parity case of the GPU tests (lx, ly, jcoup, qcoup, bdry, D, 2S)
+WELL_CONDITIONED = [(4, 2, 1.0, 0.0, 0, 100, 0), (3, 4, 2.0, 0.0, 0, 20, 2),
+                    (6, 2, 2.0, 0.0, 0, 10, 0), (4, 4, 2.0, 0.0, 0, 12, 0),
+                    (3, 4, 0.8, 0.6, 0, 20, 0), (4, 3, 0.7, 0.3, 0, 16, 2)]
+
+
+@pytest.mark.parametrize("lx,ly,jcoup,qcoup,bdry,D,twoS", WELL_CONDITIONED)
+def test_trajectory_is_well_conditioned(monkeypatch, lx, ly, jcoup, qcoup, bdry, D, twoS):
+    # The GPU run is compared with the oracle energy by energy on these
+    # lattices.  That comparison is meaningful only when the trajectory does
+    # not hinge on roundoff: perturb every Gram matrix, and every wavefunction
+    # before its SVD, by 1e-15 relative noise and require every energy to move
+    # by less than 3e-11.  (The symmetric cylinders fail this: their warm-up
+    # keeps sectors of rounding-noise weight, whose direction is the noise.)
+    lat = od.Lattice(lx, ly, jcoup=jcoup, qcoup=qcoup, bdry=bdry)
     ref = od.DMRG(lat, totspin=twoS, max_dcut=D)
     ref.run(nsweep=2, stop=None)
     eigh = np.linalg.eigh
+    svd = od.truncated_svd
     rng = np.random.default_rng(11)
 
     def noisy(G):
         E = rng.standard_normal(G.shape)
         return eigh(G + (E + E.T) * 1e-15 * np.abs(G).max())
 
+    def noisy_svd(psi, p1, p2, dcut, info=None):
+        sc = max(np.abs(b).max() for b in psi.values())
+        psi = {j: b + 1e-15 * sc * rng.standard_normal(b.shape) for j, b in psi.items()}
+        return svd(psi, p1, p2, dcut, info)
+
     monkeypatch.setattr(od.np.linalg, "eigh", noisy)
+    monkeypatch.setattr(od, "truncated_svd", noisy_svd)
     pert = od.DMRG(lat, totspin=twoS, max_dcut=D)
     pert.run(nsweep=2, stop=None)
     monkeypatch.setattr(od.np.linalg, "eigh", eigh)
-    np.testing.assert_allclose(pert.energies, ref.energies, rtol=0, atol=1e-11)
+    monkeypatch.setattr(od, "truncated_svd", svd)
+    np.testing.assert_allclose(pert.energies, ref.energies, rtol=0, atol=3e-11)
+
+
+def test_symmetric_cylinder_warmup_hinges_on_roundoff():
+    # why the cylinders are compared with exact diagonalisation instead: a
+    # 1e-15 perturbation of the wavefunctions moves a 4x3 S=1 trajectory by O(0.1)
+    lat = od.Lattice(4, 3, jcoup=0.7, qcoup=0.3, bdry=2)
+    ref = od.DMRG(lat, totspin=2, max_dcut=16)
+    ref.run(nsweep=1, stop=None)
+    svd = od.truncated_svd
+    rng = np.random.default_rng(3)
+
+    def noisy_svd(psi, p1, p2, dcut, info=None):
+        sc = max(np.abs(b).max() for b in psi.values())
+        psi = {j: b + 1e-15 * sc * rng.standard_normal(b.shape) for j, b in psi.items()}
+        return svd(psi, p1, p2, dcut, info)
+
+    od.truncated_svd = noisy_svd
+    try:
+        pert = od.DMRG(lat, totspin=2, max_dcut=16)
+        pert.run(nsweep=1, stop=None)
+    finally:
+        od.truncated_svd = svd
+    assert np.abs(np.array(pert.energies) - np.array(ref.energies)).max() > 1e-3
 
 
 def test_start_vector_recoupling_is_trivial_for_spin_half_sites():

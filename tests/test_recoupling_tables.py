@@ -52,3 +52,17 @@ def test_heff_table(j):
         if jp < 0:
             continue
         assert _close(L.recoupling("hamilt_vec", j, 2, jp), osu2.fac_hamilt_vec(j, 2, jp))
+
+
+@pytest.mark.parametrize("K", [0, 2, 4])
+def test_plaquette_coupling_tables(K):
+    n = 0
+    for jl, ja, jlp, jap in itertools.product(range(0, 11), repeat=4):
+        if abs(ja - jl) != 1 or abs(jap - jlp) != 1:
+            continue
+        for name in ("couple_left", "couple_right", "reduce_left", "reduce_right"):
+            a = L.recoupling(name, jl, ja, jlp, jap, K, 1)
+            b = getattr(orc, name)(jl, ja, jlp, jap, K, 1)
+            assert _close(a, b), (name, jl, ja, jlp, jap, K, a, b)
+            n += b != 0
+    assert n > 50
