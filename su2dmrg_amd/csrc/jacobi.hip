@@ -75,14 +75,27 @@ __global__ __launch_bounds__(NT) void gram_partial_kernel(const BlockDesc* __res
   f64x4 acc[TPW];
 #pragma unroll
   for (int j = 0; j < TPW; ++j) acc[j] = f64x4{0.0, 0.0, 0.0, 0.0};
-  __syncthreads();
-  for (int r0 = lo; r0 < hi; r0 += RCH) {
+  // the next chunk's global loads are issued before the current chunk's MFMAs
+  constexpr int LPT = RCH * NP / NT;
+  double pre[LPT];
+  auto fetch = [&](int r0) {
     const int nr = min(RCH, hi - r0);
-    for (int e = t; e < RCH * NP; e += NT) {
-      const int rr = e % RCH, c = e / RCH;
-      X[rr][c] = (rr < nr && c < n) ? B.W[(r0 + rr) + (long long)colmap[c] * B.L] : 0.0;
+#pragma unroll
+    for (int i = 0; i < LPT; ++i) {
+      const int e = t + NT * i, rr = e % RCH, c = e / RCH;
+      pre[i] = (rr < nr && c < n) ? B.W[(r0 + rr) + (long long)colmap[c] * B.L] : 0.0;
+    }
+  };
+  __syncthreads();
+  if (lo < hi) fetch(lo);
+  for (int r0 = lo; r0 < hi; r0 += RCH) {
+#pragma unroll
+    for (int i = 0; i < LPT; ++i) {
+      const int e = t + NT * i;
+      X[e % RCH][e / RCH] = pre[i];
     }
     __syncthreads();
+    if (r0 + RCH < hi) fetch(r0 + RCH);
 #pragma unroll
     for (int k4 = 0; k4 < RCH; k4 += 4) {
       const int row = k4 + (lane >> 4);
@@ -220,22 +233,33 @@ __global__ __launch_bounds__(NT) void jacobi_rotate_kernel(const BlockDesc* __re
   const double fl = 0.1 * tol_abs * B.normF;
   for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
     for (int r = 0; r < m - 1; ++r) {
-      if (t < npair) {
-        const int p = rr_p(r, t, m), q = rr_q(r, t, m);
+      // rotation of slot t: the round-robin pair for t < npair, an identity
+      // on the spare indices (2t, 2t+1) >= m otherwise, so that the HALF
+      // slots partition all NP indices and every thread below owns a fixed
+      // set of 2 x 2 blocks and R entries with no branches
+      if (t < HALF) {
+        int p, q;
         double c = 1.0, s = 0.0, tt = 0.0;
-        if (p < n && q < n) {
-          const double apq = G[p][q];
-          const double app = G[p][p], aqq = G[q][q];
-          const double thr = fmax(tol_in * sqrt(fabs(app * aqq)),
-                                  fl * (sqrt(fabs(app)) + sqrt(fabs(aqq))));
-          if (fabs(apq) > thr && fabs(apq) > 1e-300) {
-            const double tau = (aqq - app) / (2.0 * apq);
-            tt = fabs(tau) > 1e150 ? 0.5 / tau
-                                   : (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-            c = 1.0 / sqrt(1.0 + tt * tt);
-            s = tt * c;
-            rotated[sweep] = 1;
+        if (t < npair) {
+          p = rr_p(r, t, m);
+          q = rr_q(r, t, m);
+          if (p < n && q < n) {
+            const double apq = G[p][q];
+            const double app = G[p][p], aqq = G[q][q];
+            const double thr = fmax(tol_in * sqrt(fabs(app * aqq)),
+                                    fl * (sqrt(fabs(app)) + sqrt(fabs(aqq))));
+            if (fabs(apq) > thr && fabs(apq) > 1e-300) {
+              const double tau = (aqq - app) / (2.0 * apq);
+              tt = fabs(tau) > 1e150 ? 0.5 / tau
+                                     : (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+              c = 1.0 / sqrt(1.0 + tt * tt);
+              s = tt * c;
+              rotated[sweep] = 1;
+            }
           }
+        } else {
+          p = 2 * t;
+          q = 2 * t + 1;
         }
         pp[t] = p;
         qq[t] = q;
@@ -245,43 +269,25 @@ __global__ __launch_bounds__(NT) void jacobi_rotate_kernel(const BlockDesc* __re
       }
       __syncthreads();
       {
-        // G <- J^T G J on the 2 x 2 blocks (k, k2), k <= k2, mirrored to (k2,
-        // k), and R <- R J.  Every element belongs to one thread in a round,
-        // so each thread first loads all its blocks and R entries, then
-        // computes, then stores: one LDS latency per phase, not per block.
-        constexpr int KS = NT / HALF, MB = (HALF + KS - 1) / KS;
-        constexpr int RS = NT / NP, MR = (HALF + RS - 1) / RS;
+        // G <- J^T G J on the 2 x 2 blocks (k, k2) of a full-square
+        // partition (each element has one owner; identity rotations store
+        // their values back unchanged), then R <- R J.  All loads of a thread
+        // precede its stores.
+        constexpr int KS = NT / HALF, MB = HALF / KS;
+        constexpr int RS = NT / NP, MR = HALF / RS;
+        static_assert(MB * KS == HALF && MR * RS == HALF, "block partition");
         const int k = t % HALF, kb = t / HALF;
-        const bool kin = k < npair;
-        const int p = kin ? pp[k] : 0, q = kin ? qq[k] : 0;
-        const double c = kin ? cs[k] : 1.0, s = kin ? sn[k] : 0.0, tk = kin ? tn[k] : 0.0;
+        const int p = pp[k], q = qq[k];
+        const double c = cs[k], s = sn[k], tk = tn[k];
         double g[MB][4], c2v[MB], s2v[MB];
         int p2v[MB], q2v[MB];
-        int kind[MB];   // 0 nothing, 1 diagonal block, 2 off-diagonal block
 #pragma unroll
         for (int i = 0; i < MB; ++i) {
           const int k2 = kb + KS * i;
-          kind[i] = 0;
-          p2v[i] = 0;
-          q2v[i] = 0;
-          c2v[i] = 1.0;
-          s2v[i] = 0.0;
-          if (!kin || k2 >= npair || k2 < k) continue;
-          if (k2 == k) {
-            if (s == 0.0) continue;
-            kind[i] = 1;
-            g[i][0] = G[p][q];
-            g[i][1] = G[p][p];
-            g[i][2] = G[q][q];
-            continue;
-          }
-          const double s2 = sn[k2];
-          if (s == 0.0 && s2 == 0.0) continue;
-          kind[i] = 2;
-          s2v[i] = s2;
-          c2v[i] = cs[k2];
           p2v[i] = pp[k2];
           q2v[i] = qq[k2];
+          c2v[i] = cs[k2];
+          s2v[i] = sn[k2];
           g[i][0] = G[p][p2v[i]];
           g[i][1] = G[p][q2v[i]];
           g[i][2] = G[q][p2v[i]];
@@ -293,45 +299,34 @@ __global__ __launch_bounds__(NT) void jacobi_rotate_kernel(const BlockDesc* __re
 #pragma unroll
         for (int i = 0; i < MR; ++i) {
           const int kk = rb + RS * i;
-          rsn[i] = kk < npair ? sn[kk] : 0.0;
-          rc[i] = 1.0;
-          rp[i] = 0;
-          rq[i] = 0;
-          if (rsn[i] == 0.0) continue;
-          rc[i] = cs[kk];
           rp[i] = pp[kk];
           rq[i] = qq[kk];
+          rc[i] = cs[kk];
+          rsn[i] = sn[kk];
           ru[i] = R[row][rp[i]];
           rv[i] = R[row][rq[i]];
         }
 #pragma unroll
         for (int i = 0; i < MB; ++i) {
-          if (kind[i] == 1) {
-            G[p][p] = g[i][1] - tk * g[i][0];
-            G[q][q] = g[i][2] + tk * g[i][0];
-            G[p][q] = 0.0;
-            G[q][p] = 0.0;
-          } else if (kind[i] == 2) {
-            const double c2 = c2v[i], s2 = s2v[i];
-            const double t00 = c2 * g[i][0] - s2 * g[i][1], t01 = s2 * g[i][0] + c2 * g[i][1];
-            const double t10 = c2 * g[i][2] - s2 * g[i][3], t11 = s2 * g[i][2] + c2 * g[i][3];
-            const double n00 = c * t00 - s * t10, n01 = c * t01 - s * t11;
-            const double n10 = s * t00 + c * t10, n11 = s * t01 + c * t11;
-            const int p2 = p2v[i], q2 = q2v[i];
-            G[p][p2] = n00;
-            G[p][q2] = n01;
-            G[q][p2] = n10;
-            G[q][q2] = n11;
-            G[p2][p] = n00;
-            G[q2][p] = n01;
-            G[p2][q] = n10;
-            G[q2][q] = n11;
-          }
+          const double c2 = c2v[i], s2 = s2v[i];
+          const double t00 = c2 * g[i][0] - s2 * g[i][1], t01 = s2 * g[i][0] + c2 * g[i][1];
+          const double t10 = c2 * g[i][2] - s2 * g[i][3], t11 = s2 * g[i][2] + c2 * g[i][3];
+          double n00 = c * t00 - s * t10, n01 = c * t01 - s * t11;
+          double n10 = s * t00 + c * t10, n11 = s * t01 + c * t11;
+          // the rotated pair's own block: annihilated exactly, diagonal moved by t g_pq
+          const bool own = (kb + KS * i == k) && s != 0.0;
+          n00 = own ? g[i][0] - tk * g[i][1] : n00;
+          n11 = own ? g[i][3] + tk * g[i][1] : n11;
+          n01 = own ? 0.0 : n01;
+          n10 = own ? 0.0 : n10;
+          G[p][p2v[i]] = n00;
+          G[p][q2v[i]] = n01;
+          G[q][p2v[i]] = n10;
+          G[q][q2v[i]] = n11;
         }
         // (rows >= n hold zeros in the rotated columns)
 #pragma unroll
         for (int i = 0; i < MR; ++i) {
-          if (rsn[i] == 0.0) continue;
           R[row][rp[i]] = rc[i] * ru[i] - rsn[i] * rv[i];
           R[row][rq[i]] = rsn[i] * ru[i] + rc[i] * rv[i];
         }
@@ -346,46 +341,77 @@ __global__ __launch_bounds__(NT) void jacobi_rotate_kernel(const BlockDesc* __re
   // output tiles spread over the waves: A = X fragment (lane l:
   // X[16 rt + l%16][k + l/16]), B = R fragment (lane l: R[k + l/16][16 ct + l%16]).
   // The result goes back through LDS so each column segment is written whole.
+  // The W chunks of this slice and then its V chunks form one stream; the
+  // next chunk's global loads are issued before the current chunk's MFMAs.
   const int lane = t & 63, wave = t >> 6;
-  for (int which = 0; which < 2; ++which) {
-    double* M = which == 0 ? B.W : B.V;
-    const long long ld = which == 0 ? B.L : B.k;
-    int lo, hi;
-    slice_rows(which == 0 ? B.L : B.k, J.P, slice, lo, hi);
-    for (int r0 = lo; r0 < hi; r0 += RCH) {
-      const int nr = min(RCH, hi - r0);
-      __syncthreads();
-      for (int e = t; e < RCH * NP; e += NT) {
-        const int rr = e % RCH, c = e / RCH;
-        X[rr][c] = (rr < nr && c < n) ? M[(r0 + rr) + (long long)colmap[c] * ld] : 0.0;
-      }
-      __syncthreads();
-      f64x4 y[TPW];
+  constexpr int LPT = RCH * NP / NT;   // elements of a chunk per thread
+  int wlo, whi, vlo, vhi;
+  slice_rows(B.L, J.P, slice, wlo, whi);
+  slice_rows(B.k, J.P, slice, vlo, vhi);
+  const int nw = (whi - wlo + RCH - 1) / RCH, nv = (vhi - vlo + RCH - 1) / RCH;
+  auto chunk = [&](int ci, double*& M, long long& ld, int& r0, int& nr) {
+    if (ci < nw) {
+      M = B.W;
+      ld = B.L;
+      r0 = wlo + ci * RCH;
+      nr = min(RCH, whi - r0);
+    } else {
+      M = B.V;
+      ld = B.k;
+      r0 = vlo + (ci - nw) * RCH;
+      nr = min(RCH, vhi - r0);
+    }
+  };
+  double pre[LPT];
+  auto fetch = [&](int ci) {
+    double* M;
+    long long ld;
+    int r0, nr;
+    chunk(ci, M, ld, r0, nr);
 #pragma unroll
-      for (int u = 0; u < TPW; ++u) y[u] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < LPT; ++i) {
+      const int e = t + NT * i, rr = e % RCH, c = e / RCH;
+      pre[i] = (rr < nr && c < n) ? M[(r0 + rr) + (long long)colmap[c] * ld] : 0.0;
+    }
+  };
+  if (nw + nv > 0) fetch(0);
+  for (int ci = 0; ci < nw + nv; ++ci) {
+    double* M;
+    long long ld;
+    int r0, nr;
+    chunk(ci, M, ld, r0, nr);
+    __syncthreads();
 #pragma unroll
-      for (int k = 0; k < NP; k += 4) {
+    for (int i = 0; i < LPT; ++i) {
+      const int e = t + NT * i;
+      X[e % RCH][e / RCH] = pre[i];
+    }
+    __syncthreads();
+    if (ci + 1 < nw + nv) fetch(ci + 1);
+    f64x4 y[TPW];
 #pragma unroll
-        for (int u = 0; u < TPW; ++u) {
-          const int tau = wave * TPW + u, rt = tau % RT, ct = tau / RT;
-          y[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(X[16 * rt + (lane & 15)][k + (lane >> 4)],
-                                                      R[k + (lane >> 4)][16 * ct + (lane & 15)],
-                                                      y[u], 0, 0, 0);
-        }
-      }
-      __syncthreads();
+    for (int u = 0; u < TPW; ++u) y[u] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < NP; k += 4) {
 #pragma unroll
       for (int u = 0; u < TPW; ++u) {
         const int tau = wave * TPW + u, rt = tau % RT, ct = tau / RT;
+        y[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(X[16 * rt + (lane & 15)][k + (lane >> 4)],
+                                                    R[k + (lane >> 4)][16 * ct + (lane & 15)],
+                                                    y[u], 0, 0, 0);
+      }
+    }
+    __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-          X[16 * rt + (lane >> 4) + 4 * q][16 * ct + (lane & 15)] = y[u][q];
-      }
-      __syncthreads();
-      for (int e = t; e < RCH * n; e += NT) {
-        const int rr = e % RCH, c = e / RCH;
-        if (rr < nr) M[(r0 + rr) + (long long)colmap[c] * ld] = X[rr][c];
-      }
+    for (int u = 0; u < TPW; ++u) {
+      const int tau = wave * TPW + u, rt = tau % RT, ct = tau / RT;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) X[16 * rt + (lane >> 4) + 4 * q][16 * ct + (lane & 15)] = y[u][q];
+    }
+    __syncthreads();
+    for (int e = t; e < RCH * n; e += NT) {
+      const int rr = e % RCH, c = e / RCH;
+      if (rr < nr) M[(r0 + rr) + (long long)colmap[c] * ld] = X[rr][c];
     }
   }
 }
