@@ -126,6 +126,13 @@ int su2_heff_autotune(su2_ctx* ctx, su2_heff* plan, int reps, int* chosen, float
                       int* nvariants);
 /* Pin the tile variants of the two launches (e.g. to a tuned choice). */
 int su2_heff_set_variants(su2_ctx* ctx, su2_heff* plan, int stage1, int stage2);
+/* Kernel path of the two launches: uniform = 1 (default) runs a launch whose
+ * slices share one operand layout (stage 1: op(A) = A^T, stage 2: op(A) = A)
+ * with the straight-line kernel, 0 with the general kernel; both give the
+ * same bits.  layouts (nullable) receives the layout of each stage: 0 for
+ * op(A) = A, 1 for op(A) = A^T, -1 for the general kernel.  No counterpart in
+ * the reference. */
+int su2_heff_set_kernel_path(su2_ctx* ctx, su2_heff* plan, int uniform, int* layouts);
 
 /* Truncated SVD of a two-leg singlet wavefunction across sectors
  * (tensor_su2::svd(tu,p1,tv,p2,wout), tensor_su2.cpp:1619-1758, with

@@ -399,6 +399,18 @@ int su2_heff_set_variants(su2_ctx* ctx, su2_heff* plan, int stage1, int stage2) 
   });
 }
 
+int su2_heff_set_kernel_path(su2_ctx* ctx, su2_heff* plan, int uniform, int* layouts) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(plan, "plan");
+    plan->p.set_uniform(uniform != 0);
+    if (layouts) {
+      layouts[0] = plan->p.stage1().layout();
+      layouts[1] = plan->p.stage2().layout();
+    }
+  });
+}
+
 int su2_svd_truncate(su2_ctx* ctx, const su2_tensor* psi, double p1, double p2, int max_dcut,
                      su2_tensor** U, su2_tensor** V, double* kept, int cap, int* nkept,
                      double* discarded) {

@@ -260,10 +260,19 @@ __device__ __forceinline__ void store_slab(const GemmSlice& s,
   }
 }
 
-// Two waves per SIMD (at most 256 registers per lane): one wave's MFMAs run
-// while the other loads, stores its next slab or waits at the barrier.
+// Waves per SIMD: two for sub-tiles up to 64 x 64 per wave (at most 256
+// registers per lane; one wave's MFMAs run while the other loads, stores its
+// next slab or waits at the barrier), one for larger sub-tiles, whose
+// accumulators take the 256 accumulation registers and whose MFMA work per
+// slab covers the barrier by itself.
+template <int BM, int BN, int WM, int WN>
+struct WavesPerSimd {
+  static constexpr int value = (BM / WM) * (BN / WN) > 64 * 64 ? 1 : 2;
+};
+
 template <int BM, int BN, int BK, int WM, int WN, int PAD>
-__global__ __launch_bounds__(WM * WN * 64, 2) void grouped_gemm_f64_kernel(
+__global__ __launch_bounds__(WM * WN * 64, (WavesPerSimd<BM, BN, WM, WN>::value)) void
+grouped_gemm_f64_kernel(
     const GemmProblem* __restrict__ probs, int nprob, const GemmSlice* __restrict__ slices,
     double* __restrict__ ws, long long total_units) {
   using T = Tile<BM, BN, BK, WM, WN, PAD>;
@@ -372,6 +381,209 @@ __global__ __launch_bounds__(WM * WN * 64, 2) void grouped_gemm_f64_kernel(
       }
 }
 
+// ---------------------------------------------------------------------------
+// Uniform-layout kernel: every slice of the launch has op(B) = B (stored
+// K x N) and the same op(A) (TA = 0: A stored M x K; TA = 1: A stored K x M,
+// used transposed).  With the layout fixed at compile time, a full slab's
+// loads and LDS stores are straight-line code.  Elements past the M / N edge
+// load a clamped in-matrix element rather than being masked: they only reach
+// tile rows and columns that the epilogue does not store.  Only the K tail
+// of a slice is zeroed, since its extra products would reach stored outputs;
+// its out-of-range elements load a zero from device memory, so no value
+// waits on its load.  The next slab's scale-and-store to LDS is issued
+// inside the last group of MFMAs of the current slab, so the matrix cores
+// keep working while it runs, and only the barrier separates two slabs.
+// (Unrolling the K loop by two, to make the buffers compile-time, makes the
+// register allocator spill the accumulators; the buffer is a runtime select.)
+
+__device__ double g_gemm_zero = 0.0;
+
+template <int BM, int BN, int BK, int WM, int WN, int PAD, int TA>
+__device__ __forceinline__ void load_full_u(const GemmSlice& s, const SliceAddr& ad, int k0,
+                                            SlabRegs<Tile<BM, BN, BK, WM, WN, PAD>>& r) {
+  using T = Tile<BM, BN, BK, WM, WN, PAD>;
+  const unsigned lda8 = 8u * (unsigned)s.lda, ldb8 = 8u * (unsigned)s.ldb;
+  const unsigned kk = (unsigned)k0;
+  if (TA == 0) {
+    const unsigned base = ad.a0 + kk * lda8, step = (T::NT / BM) * lda8;
+#pragma unroll
+    for (int i = 0; i < T::APT; ++i) r.a[i] = gload_b(s.A, base + i * step);
+  } else {
+    const unsigned step = (T::NT / BK) * lda8;
+#pragma unroll
+    for (int i = 0; i < T::APT; ++i)
+      r.a[i] = gload_b(s.A, (((ad.amn >> i) & 1u) ? ad.a0 + i * step : ad.a_safe) + 8u * kk);
+  }
+  const unsigned step = (T::NT / BK) * ldb8;
+#pragma unroll
+  for (int i = 0; i < T::BPT; ++i)
+    r.b[i] = gload_b(s.B, (((ad.bmn >> i) & 1u) ? ad.b0 + i * step : ad.b_safe) + 8u * kk);
+}
+
+// The last, partial slab of a slice: elements with k >= K read g_gemm_zero.
+template <int BM, int BN, int BK, int WM, int WN, int PAD, int TA>
+__device__ __forceinline__ void load_tail_u(const GemmSlice& s, int k0, int m0, int n0, int M,
+                                            int N, SlabRegs<Tile<BM, BN, BK, WM, WN, PAD>>& r) {
+  using T = Tile<BM, BN, BK, WM, WN, PAD>;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < T::APT; ++i) {
+    const int e = t + i * T::NT;
+    const int gm = min(m0 + (TA == 0 ? e % BM : e / BK), M - 1);
+    const int gk = k0 + (TA == 0 ? e / BM : e % BK);
+    const double* p = TA == 0 ? s.A + gm + (long long)gk * s.lda : s.A + gk + (long long)gm * s.lda;
+    r.a[i] = *(gdptr)(gk < s.K ? p : &g_gemm_zero);
+  }
+#pragma unroll
+  for (int i = 0; i < T::BPT; ++i) {
+    const int e = t + i * T::NT;
+    const int gn = min(n0 + e / BK, N - 1);
+    const int gk = k0 + e % BK;
+    const double* p = s.B + gk + (long long)gn * s.ldb;
+    r.b[i] = *(gdptr)(gk < s.K ? p : &g_gemm_zero);
+  }
+}
+
+template <int BM, int BN, int BK, int WM, int WN, int PAD, int TA>
+__device__ __forceinline__ void store_u(const SlabRegs<Tile<BM, BN, BK, WM, WN, PAD>>& r,
+                                        double alpha, double* As, double* Bs) {
+  using T = Tile<BM, BN, BK, WM, WN, PAD>;
+  const int t = threadIdx.x;
+  if (TA == 0) {
+    double* a = As + (t / BM) * T::LDA_S + t % BM;
+#pragma unroll
+    for (int i = 0; i < T::APT; ++i) a[i * (T::NT / BM) * T::LDA_S] = alpha * r.a[i];
+  } else {
+    double* a = As + (t % BK) * T::LDA_S + t / BK;
+#pragma unroll
+    for (int i = 0; i < T::APT; ++i) a[i * (T::NT / BK)] = alpha * r.a[i];
+  }
+  double* b = Bs + (t % BK) * T::LDB_S + t / BK;
+#pragma unroll
+  for (int i = 0; i < T::BPT; ++i) b[i * (T::NT / BK)] = r.b[i];
+}
+
+template <int BM, int BN, int BK, int WM, int WN, int PAD, int TA>
+__global__ __launch_bounds__(WM * WN * 64, (WavesPerSimd<BM, BN, WM, WN>::value)) void
+grouped_gemm_f64_uniform(const GemmProblem* __restrict__ probs, int nprob,
+                         const GemmSlice* __restrict__ slices, double* __restrict__ ws,
+                         long long total_units) {
+  using T = Tile<BM, BN, BK, WM, WN, PAD>;
+  __shared__ double As0[BK * T::LDA_S], As1[BK * T::LDA_S];
+  __shared__ double Bs0[BK * T::LDB_S], Bs1[BK * T::LDB_S];
+
+  const long long unit = xcd_remap(blockIdx.x, gridDim.x);
+  if (unit >= total_units) return;
+  const int p = find_problem(probs, nprob, unit);
+  const GemmProblem P = probs[p];
+  long long u = unit - P.unit_begin;
+  const int ntile = P.tiles_m * P.tiles_n;
+  const int ks = (int)(u / ntile);
+  const int tile = (int)(u % ntile);
+  const int tm = P.n_fast ? tile / P.tiles_n : tile % P.tiles_m;
+  const int tn = P.n_fast ? tile % P.tiles_n : tile / P.tiles_m;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int kt_lo = (int)((long long)P.kt_total * ks / P.ksplit);
+  const int kt_hi = (int)((long long)P.kt_total * (ks + 1) / P.ksplit);
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm0 = (wave % WM) * T::WTM;
+  const int wn0 = (wave / WM) * T::WTN;
+
+  f64x4 acc[T::FM][T::FN];
+#pragma unroll
+  for (int i = 0; i < T::FM; ++i)
+#pragma unroll
+    for (int j = 0; j < T::FN; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+  if (kt_lo < kt_hi) {
+    int s = P.slice_begin;
+    while (s + 1 < P.slice_end && slices[s + 1].kt_begin <= kt_lo) ++s;
+    GemmSlice S = slices[s];
+    int kl = kt_lo - S.kt_begin;
+    int nkt = (S.K + BK - 1) / BK;
+    SliceAddr ad = slice_addr<BM, BN, BK, WM, WN, PAD>(S, m0, n0, P.M, P.N);
+    SlabRegs<T> regs;
+    auto load = [&]() {
+      if ((kl + 1) * BK <= S.K) load_full_u<BM, BN, BK, WM, WN, PAD, TA>(S, ad, kl * BK, regs);
+      else load_tail_u<BM, BN, BK, WM, WN, PAD, TA>(S, kl * BK, m0, n0, P.M, P.N, regs);
+    };
+    load();
+    store_u<BM, BN, BK, WM, WN, PAD, TA>(regs, S.alpha, As0, Bs0);
+    __syncthreads();
+
+    // one slab from buffer CUR; the next slab (if any) is fetched into
+    // registers first and stored to the other buffer before the last MFMAs
+    auto step = [&](int cur, bool more) {
+      const double* Ab = cur ? As1 : As0;
+      const double* Bb = cur ? Bs1 : Bs0;
+      double* An = cur ? As0 : As1;
+      double* Bn = cur ? Bs0 : Bs1;
+      if (more) {
+        if (++kl >= nkt) {
+          S = slices[++s];
+          kl = 0;
+          nkt = (S.K + BK - 1) / BK;
+          ad = slice_addr<BM, BN, BK, WM, WN, PAD>(S, m0, n0, P.M, P.N);
+        }
+        load();
+      }
+      // Fragments of k-step kk + 1 are read while the MFMAs of kk run (two
+      // register sets); the scheduling barriers keep the scheduler from
+      // hoisting every step's reads to the top.  With no next slab the store
+      // rewrites stale registers into the idle buffer, which no wave reads
+      // again.
+      double af[2][T::FM], bf[2][T::FN];
+      auto frags = [&](int kk, int set) {
+        const int kr = kk + (lane >> 4);
+#pragma unroll
+        for (int i = 0; i < T::FM; ++i)
+          af[set][i] = Ab[kr * T::LDA_S + wm0 + i * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < T::FN; ++j)
+          bf[set][j] = Bb[kr * T::LDB_S + wn0 + j * 16 + (lane & 15)];
+      };
+      frags(0, 0);
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        const int set = (kk / 4) & 1;
+        if (kk + 4 < BK) frags(kk + 4, set ^ 1);
+        if (kk == BK - 4) store_u<BM, BN, BK, WM, WN, PAD, TA>(regs, S.alpha, An, Bn);
+#pragma unroll
+        for (int i = 0; i < T::FM; ++i)
+#pragma unroll
+          for (int j = 0; j < T::FN; ++j)
+            acc[i][j] =
+                __builtin_amdgcn_mfma_f64_16x16x4f64(af[set][i], bf[set][j], acc[i][j], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      __syncthreads();
+    };
+    for (int kt = kt_lo; kt < kt_hi; ++kt) step((kt - kt_lo) & 1, kt + 1 < kt_hi);
+  }
+
+  double* dst;
+  long long ld;
+  if (P.ksplit == 1) {
+    dst = P.C;
+    ld = P.ldc;
+  } else {
+    dst = ws + P.ws_offset + (long long)ks * P.M * P.N;
+    ld = P.M;
+  }
+#pragma unroll
+  for (int i = 0; i < T::FM; ++i)
+#pragma unroll
+    for (int j = 0; j < T::FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int row = m0 + wm0 + i * 16 + (lane >> 4) + 4 * r;
+        int col = n0 + wn0 + j * 16 + (lane & 15);
+        if (row < P.M && col < P.N) dst[row + (long long)col * ld] = acc[i][j][r];
+      }
+}
+
 __global__ void splitk_reduce_kernel(const ReduceProblem* __restrict__ reds, int nred,
                                      const double* __restrict__ ws, long long total) {
   long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -395,12 +607,15 @@ __global__ void splitk_reduce_kernel(const ReduceProblem* __restrict__ reds, int
 typedef void (*GemmKernel)(const GemmProblem*, int, const GemmSlice*, double*, long long);
 
 struct VariantInfo {
-  GemmKernel fn;
+  GemmKernel fn;   // any mix of slice layouts
+  GemmKernel fn_nn, fn_tn;  // every slice op(B) = B and op(A) = A / A^T
   int bm, bn, bk, threads;
 };
 
-#define SU2_VARIANT(BM, BN, BK, WM, WN, PAD) \
-  {grouped_gemm_f64_kernel<BM, BN, BK, WM, WN, PAD>, BM, BN, BK, WM * WN * 64}
+#define SU2_VARIANT(BM, BN, BK, WM, WN, PAD)                                               \
+  {grouped_gemm_f64_kernel<BM, BN, BK, WM, WN, PAD>,                                       \
+   grouped_gemm_f64_uniform<BM, BN, BK, WM, WN, PAD, 0>,                                   \
+   grouped_gemm_f64_uniform<BM, BN, BK, WM, WN, PAD, 1>, BM, BN, BK, WM * WN * 64}
 
 const VariantInfo kVariants[] = {
     SU2_VARIANT(128, 128, 16, 2, 2, 0),   // 0
@@ -410,6 +625,8 @@ const VariantInfo kVariants[] = {
     SU2_VARIANT(64, 64, 16, 2, 2, 2),     // 4
     SU2_VARIANT(128, 128, 32, 2, 2, 2),   // 5
     SU2_VARIANT(128, 64, 16, 2, 2, 2),    // 6
+    SU2_VARIANT(256, 128, 16, 2, 2, 2),   // 7: 128 x 64 per wave, one wave per SIMD
+    SU2_VARIANT(128, 256, 16, 2, 2, 2),   // 8: 64 x 128 per wave
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -490,6 +707,16 @@ void GroupedGemm::set_variant(int v) {
   const VariantInfo& V = kVariants[v];
   release();
   variant_ = v;
+  // one layout for every slice selects the straight-line kernel
+  layout_ = -1;
+  if (uniform_ && !slices_.empty()) {
+    bool nn = true, tn = true;
+    for (auto& x : slices_) {
+      nn = nn && x.transB == 0 && x.transA == 0;
+      tn = tn && x.transB == 0 && x.transA == 1;
+    }
+    layout_ = nn ? 0 : tn ? 1 : -1;
+  }
   std::vector<GemmSlice> sl = slices_;
   double work = 0;
   std::vector<GemmProblem> probs = probs_;
@@ -557,7 +784,8 @@ void GroupedGemm::launch(hipStream_t stream) const {
   SU2_CHECK(variant_ >= 0, kErrArgument, "gemm plan not finalized");
   if (total_units_ == 0) return;
   const VariantInfo& V = kVariants[variant_];
-  hipLaunchKernelGGL(V.fn, dim3((unsigned)total_units_), dim3(V.threads), 0, stream, d_probs_,
+  const GemmKernel fn = layout_ == 0 ? V.fn_nn : layout_ == 1 ? V.fn_tn : V.fn;
+  hipLaunchKernelGGL(fn, dim3((unsigned)total_units_), dim3(V.threads), 0, stream, d_probs_,
                      (int)run_probs_.size(), d_slices_, d_ws_, total_units_);
   SU2_HIP(hipGetLastError());
   if (!reds_.empty()) {

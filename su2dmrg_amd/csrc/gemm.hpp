@@ -69,6 +69,11 @@ class GroupedGemm {
   // Re-tiles for another variant (device tables rebuilt).
   void set_variant(int v);
   int variant() const { return variant_; }
+  // Launches whose slices share one layout (op(B) = B, op(A) fixed) run the
+  // straight-line uniform kernel unless this is switched off (before
+  // finalize or set_variant); layout() is 0 (NN), 1 (TN) or -1 (mixed).
+  void allow_uniform(bool on) { uniform_ = on; }
+  int layout() const { return layout_; }
   // Times every variant (HIP events, `reps` launches each) and keeps the
   // fastest; returns it.  `times` receives ms per launch by variant.
   int autotune(hipStream_t stream, int reps, std::vector<float>* times = nullptr);
@@ -94,6 +99,8 @@ class GroupedGemm {
   double flops_ = 0;
   int target_units_ = 2048;
   int variant_ = -1;
+  int layout_ = -1;
+  bool uniform_ = true;
 };
 
 }  // namespace su2

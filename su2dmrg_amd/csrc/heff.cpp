@@ -139,6 +139,17 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
     tr.x_off = work;
     work += (long long)tr.Dap * tr.nt * tr.Db;
   }
+  // H_L of each sector transposed: every stage-2 slice is then op(A) = A,
+  // one layout, which the grouped GEMM runs with its straight-line kernel
+  std::map<int, long long> hlt_off;
+  if (hl)
+    for (auto& b : psi.blocks()) {
+      const int j = b.twoj[0];
+      if (hl->find2(j, j) >= 0 && !hlt_off.count(j)) {
+        hlt_off[j] = work;
+        work += (long long)b.dim[0] * b.dim[0];
+      }
+    }
   SU2_HIP(hipMalloc(&d_in_, std::max<long long>(psi_size_, 1) * sizeof(double)));
   SU2_HIP(hipMalloc(&d_out_, std::max<long long>(psi_size_, 1) * sizeof(double)));
   if (work > 0) SU2_HIP(hipMalloc(&d_work_, work * sizeof(double)));
@@ -204,6 +215,34 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
     SU2_HIP(hipFree(d_desc));
   }
 
+  if (!hlt_off.empty()) {
+    std::vector<StridedAxpy> desc;
+    long long total = 0;
+    for (auto& kv : hlt_off) {
+      const Block& hb = hl->blocks()[hl->find2(kv.first, kv.first)];
+      const int D = hb.dim[0];
+      StridedAxpy d{};
+      d.dst = d_work_ + kv.second;
+      d.src = hl->data() + hb.offset;
+      d.rs = D;  // source row r lands in column r
+      d.cs = 1;
+      d.ld = D;
+      d.rows = D;
+      d.cols = D;
+      d.alpha = 1.0;
+      d.elem_begin = total;
+      total += (long long)D * D;
+      desc.push_back(d);
+    }
+    StridedAxpy* d_desc = nullptr;
+    SU2_HIP(hipMalloc(&d_desc, desc.size() * sizeof(StridedAxpy)));
+    SU2_HIP(hipMemcpyAsync(d_desc, desc.data(), desc.size() * sizeof(StridedAxpy),
+                           hipMemcpyHostToDevice, c.stream()));
+    dev_strided_axpy(c, d_desc, (int)desc.size(), total);
+    SU2_HIP(hipStreamSynchronize(c.stream()));
+    SU2_HIP(hipFree(d_desc));
+  }
+
   // stage 1: X(j->j') = f * Lb^T psi(j)   [T*D_a(j') x D_b(j)]
   for (auto& tr : trs) {
     const Block* bj = pb[tr.ja];
@@ -228,13 +267,9 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
       stage2_.add_slice(p, d_work_ + tr.x_off, Dap, 0, d_work_ + tr.rb_off,
                         (long long)tr.nt * tr.Db, 0, tr.nt * tr.Db, 1.0);
     }
-    if (hl) {
-      int id = hl->find2(jap, jap);
-      if (id >= 0) {
-        ensure();
-        stage2_.add_slice(p, hl->data() + hl->blocks()[id].offset, Dap, 1, d_in_ + b.offset, Dap,
-                          0, Dap, 1.0);
-      }
+    if (hlt_off.count(jap)) {
+      ensure();
+      stage2_.add_slice(p, d_work_ + hlt_off.at(jap), Dap, 0, d_in_ + b.offset, Dap, 0, Dap, 1.0);
     }
     if (hr) {
       int id = hr->find2(b.twoj[1], b.twoj[1]);
@@ -270,6 +305,14 @@ void HeffPlan::set_variants(int v1, int v2) {
   ctx_.make_current();
   stage1_.set_variant(v1);
   stage2_.set_variant(v2);
+}
+
+void HeffPlan::set_uniform(bool on) {
+  ctx_.make_current();
+  stage1_.allow_uniform(on);
+  stage2_.allow_uniform(on);
+  stage1_.set_variant(stage1_.variant());
+  stage2_.set_variant(stage2_.variant());
 }
 
 void HeffPlan::autotune(int reps, int* chosen, std::vector<float>* t1, std::vector<float>* t2) {

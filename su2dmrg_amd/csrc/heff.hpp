@@ -57,6 +57,10 @@ class HeffPlan {
   // Tile-variant choice of the two grouped launches: set explicitly, or time
   // every compiled variant on this plan's own problems and keep the fastest.
   void set_variants(int v1, int v2);
+  // Straight-line GEMM kernel for the single-layout launches (default) or
+  // the general kernel; both give the same bits.  Returns nothing; the
+  // layouts in use are stage1().layout(), stage2().layout().
+  void set_uniform(bool on);
   void autotune(int reps, int* chosen, std::vector<float>* t1, std::vector<float>* t2);
 
  private:

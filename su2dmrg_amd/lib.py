@@ -57,6 +57,7 @@ _SIG = {
     "su2_heff_time": (_i, [_p, _p, _p, _p, _i, _dp, _dp, _dp]),
     "su2_heff_autotune": (_i, [_p, _p, _i, _ip, C.POINTER(C.c_float), _ip]),
     "su2_heff_set_variants": (_i, [_p, _p, _i, _i]),
+    "su2_heff_set_kernel_path": (_i, [_p, _p, _i, _p]),
     "su2_svd_truncate": (_i, [_p, _p, _d, _d, _i, C.POINTER(_p), C.POINTER(_p), _dp, _i, _ip,
                               _dp]),
     "su2_recoupling": (_i, [_i, _ip, _dp]),
@@ -321,6 +322,13 @@ class HeffPlan:
 
     def set_variants(self, v1, v2):
         self.ctx.check(lib.su2_heff_set_variants(self.ctx.h, self.h, int(v1), int(v2)))
+
+    def set_kernel_path(self, uniform):
+        """Straight-line kernel for single-layout launches (True) or the
+        general kernel (False); returns the (stage 1, stage 2) layouts."""
+        lay = (C.c_int * 2)()
+        self.ctx.check(lib.su2_heff_set_kernel_path(self.ctx.h, self.h, int(bool(uniform)), lay))
+        return lay[0], lay[1]
 
     def close(self):
         if self.h:

@@ -128,6 +128,34 @@ def test_every_tile_variant_matches_oracle(ctx):
         assert _relerr(got, ref) < RTOL, v
 
 
+def test_uniform_and_general_gemm_kernels_agree_bitwise(ctx):
+    # stage 1 is all op(A) = A^T, stage 2 all op(A) = A (H_L is transposed at
+    # plan build), so both launches take the straight-line kernel; it stages
+    # the same values through LDS in the same order as the general kernel,
+    # so the two agree to the bit for every tile shape (M / N edges, K tails
+    # and split K included by these dimensions)
+    from su2dmrg_amd import lib as L
+    _say("heff uniform vs general GEMM kernel")
+    work = synthetic.small(seed=21, twoj=(1, 3, 5, 7), dims=(150, 290, 77, 3), nterm=3)
+    dev = L.upload_workload(ctx, work)
+    ref = oheff.heff_apply(work["psi"], work["hl"], work["hr"], work["terms"])
+    plan = dev["plan"]
+    _, t1, _ = plan.autotune(reps=1)
+    for v in range(len(t1)):
+        _say(f"  variant {v}")
+        plan.set_variants(v, v)
+        assert plan.set_kernel_path(True) == (1, 0)
+        plan.apply(dev["psi"], dev["out"])
+        a = dev["out"].download()
+        assert plan.set_kernel_path(False) == (-1, -1)
+        plan.apply(dev["psi"], dev["out"])
+        b = dev["out"].download()
+        assert np.array_equal(a, b), v
+        got = L.unpack_wavefunction(dev["out"], a)
+        assert _relerr(got, ref) < RTOL, v
+    plan.set_kernel_path(True)
+
+
 def test_vector_algebra(ctx):
     from su2dmrg_amd import lib as L
     _say("vector algebra")
