@@ -556,6 +556,25 @@ grouped_gemm_f64_uniform(const GemmProblem* __restrict__ probs, int nprob,
           for (int j = 0; j < T::FN; ++j)
             acc[i][j] =
                 __builtin_amdgcn_mfma_f64_16x16x4f64(af[set][i], bf[set][j], acc[i][j], 0, 0, 0);
+        // issue order inside the step (masks: 0x8 MFMA, 0x100 LDS read,
+        // 0x2 VALU, 0x200 LDS write): the next fragments right after the
+        // first MFMA, so they have the whole group to arrive; in the last
+        // step the scale-and-store is spread one MFMA apart
+        if (kk + 4 < BK) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+          __builtin_amdgcn_sched_group_barrier(0x100, (T::FM + T::FN + 1) / 2, 0);
+          __builtin_amdgcn_sched_group_barrier(0x008, T::FM * T::FN - 1, 0);
+        } else {
+          constexpr int NQ = T::FM * T::FN < 8 ? T::FM * T::FN : 8;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
+            __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+          }
+          if (T::FM * T::FN > NQ)
+            __builtin_amdgcn_sched_group_barrier(0x008, T::FM * T::FN - NQ, 0);
+        }
         __builtin_amdgcn_sched_barrier(0);
       }
       __syncthreads();
