@@ -40,6 +40,21 @@ __device__ __forceinline__ int rr_q(int r, int k, int m) {
   return k == 0 ? r : (r - k + (m - 1)) % (m - 1);
 }
 
+// 1 / x and 1 / sqrt(x) for x > 0 (normal): the hardware estimates
+// refined by two Newton-Raphson steps, y <- y (2 - x y) and
+// y <- y (3 - x y^2) / 2, each of which squares the relative error.
+__device__ __forceinline__ double recip_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return fma(y, fma(-x, y, 1.0), y);
+}
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  y = y * fma(-hx * y, y, 1.5);
+  return y * fma(-hx * y, y, 1.5);
+}
+
 // rows [lo, hi) of slice s when L rows are cut into P slices
 __device__ __forceinline__ void slice_rows(int L, int P, int s, int& lo, int& hi) {
   const int per = (L + P - 1) / P;
@@ -132,8 +147,11 @@ __global__ __launch_bounds__(NT) void gram_partial_kernel(const BlockDesc* __res
 // bound by its VALU instructions).  A round updates G <- J^T G J by 2 x 2 blocks, one
 // thread per block pair, with the rotated entry set to zero and the diagonal
 // moved by t g_pq; a rotation is skipped below max(tol_in sqrt(g_pp g_qq),
-// 0.1 tol_abs ||A||_F (sqrt g_pp + sqrt g_qq)), ten times below the outer
-// test.
+// 0.2 tol_abs ||A||_F sqrt(max(g_pp, g_qq))), about ten times below the
+// outer test.  Round latency, not arithmetic, bounds this loop (a 32 x 32
+// round costs about as much as a 64 x 64 one), so the rotation parameters
+// avoid the correctly rounded square root and division sequences and the
+// 64 x 64 case runs 512 threads (two waves per SIMD).
 template <int CB, int NT>
 __global__ __launch_bounds__(NT) void jacobi_rotate_kernel(const BlockDesc* __restrict__ blocks,
                                                            const PairJob* __restrict__ jobs,
@@ -245,14 +263,23 @@ __global__ __launch_bounds__(NT) void jacobi_rotate_kernel(const BlockDesc* __re
           q = rr_q(r, t, m);
           if (p < n && q < n) {
             const double apq = G[p][q];
-            const double app = G[p][p], aqq = G[q][q];
-            const double thr = fmax(tol_in * sqrt(fabs(app * aqq)),
-                                    fl * (sqrt(fabs(app)) + sqrt(fabs(aqq))));
-            if (fabs(apq) > thr && fabs(apq) > 1e-300) {
-              const double tau = (aqq - app) / (2.0 * apq);
-              tt = fabs(tau) > 1e150 ? 0.5 / tau
-                                     : (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-              c = 1.0 / sqrt(1.0 + tt * tt);
+            const double gpp = G[p][p], gqq = G[q][q];
+            const double app = fabs(gpp), aqq = fabs(gqq);
+            // skip test on squares (no square roots): |g_pq| against
+            // tol_in sqrt(g_pp g_qq) and the rounding floor 2 fl sqrt(max g)
+            const double a2 = apq * apq;
+            if (a2 > tol_in * tol_in * (app * aqq) && a2 > 4.0 * fl * fl * fmax(app, aqq) &&
+                fabs(apq) > 1e-300) {
+              // t = tan(theta) of the smaller rotation angle,
+              //   t = 2 g_pq sign(d) / (|d| + sqrt(d^2 + 4 g_pq^2)), d = g_qq - g_pp
+              // (the textbook tau form without its division); c = 1 / sqrt(1 +
+              // t^2) keeps the rotation orthogonal to rounding for any t, so
+              // the reciprocals need not be correctly rounded
+              const double d = gqq - gpp;
+              const double h2 = d * d + 4.0 * a2;
+              const double den = fabs(d) + h2 * rsqrt_nr(h2);
+              tt = (d >= 0 ? 2.0 : -2.0) * apq * recip_nr(den);
+              c = rsqrt_nr(1.0 + tt * tt);
               s = tt * c;
               rotated[sweep] = 1;
             }
@@ -674,7 +701,7 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
       } else {
         hipLaunchKernelGGL((gram_partial_kernel<32, 256>), dim3(nj * pm), dim3(256), 0, st,
                            (const BlockDesc*)ddesc.p, jr, pm, (double*)dgpart.p);
-        hipLaunchKernelGGL((jacobi_rotate_kernel<32, 256>), dim3(nj * pm), dim3(256), 0, st,
+        hipLaunchKernelGGL((jacobi_rotate_kernel<32, 512>), dim3(nj * pm), dim3(512), 0, st,
                            (const BlockDesc*)ddesc.p, jr, pm, (const double*)dgpart.p, tol, 1e-14,
                            (unsigned long long*)doff.p, inner_sweeps);
       }
