@@ -161,14 +161,17 @@ __global__ __launch_bounds__(NT) void jacobi_rotate_kernel(const BlockDesc* __re
                                                            unsigned long long* __restrict__ off,
                                                            int inner_sweeps) {
   constexpr int NP = 2 * CB, HALF = NP / 2, TT = NP / 16, NW = NT / 64;
+  // 512 and more threads: G and R are updated by separate halves
+  constexpr bool SPLIT = NT >= 512;
+  constexpr int NTG = SPLIT ? NT / 2 : NT, NTR = SPLIT ? NT / 2 : NT;
   constexpr int RT = RCH / 16, TPW = RT * TT / NW;
   static_assert(TPW * NW == RT * TT, "tiles must split evenly over the waves");
   static_assert(NT >= HALF && NT % NP == 0, "thread mapping");
   __shared__ double G[NP][NP + 1];
   __shared__ double R[NP][NP + 1];
   __shared__ double X[RCH][NP + 1];
-  __shared__ double cs[HALF], sn[HALF], tn[HALF];
-  __shared__ int pp[HALF], qq[HALF];
+  __shared__ double cs[2][HALF], sn[2][HALF], tn[2][HALF];
+  __shared__ int pp[2][HALF], qq[2][HALF];
   __shared__ int colmap[NP];
   __shared__ double red[NT];
   __shared__ int rotated[INNER_SWEEPS];
@@ -247,120 +250,151 @@ __global__ __launch_bounds__(NT) void jacobi_rotate_kernel(const BlockDesc* __re
   // ---- eigenvectors of G by parallel cyclic Jacobi ----
   const int m = n + (n & 1);   // index n (odd n) is a zero row and column
   const int npair = m / 2;
+  const int nround = m - 1;
   const double tol_in = 1e-14;
   const double fl = 0.1 * tol_abs * B.normF;
-  for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
-    for (int r = 0; r < m - 1; ++r) {
-      // rotation of slot t: the round-robin pair for t < npair, an identity
-      // on the spare indices (2t, 2t+1) >= m otherwise, so that the HALF
-      // slots partition all NP indices and every thread below owns a fixed
-      // set of 2 x 2 blocks and R entries with no branches
-      if (t < HALF) {
-        int p, q;
-        double c = 1.0, s = 0.0, tt = 0.0;
-        if (t < npair) {
-          p = rr_p(r, t, m);
-          q = rr_q(r, t, m);
-          if (p < n && q < n) {
-            const double apq = G[p][q];
-            const double gpp = G[p][p], gqq = G[q][q];
-            const double app = fabs(gpp), aqq = fabs(gqq);
-            // skip test on squares (no square roots): |g_pq| against
-            // tol_in sqrt(g_pp g_qq) and the rounding floor 2 fl sqrt(max g)
-            const double a2 = apq * apq;
-            if (a2 > tol_in * tol_in * (app * aqq) && a2 > 4.0 * fl * fl * fmax(app, aqq) &&
-                fabs(apq) > 1e-300) {
-              // t = tan(theta) of the smaller rotation angle,
-              //   t = 2 g_pq sign(d) / (|d| + sqrt(d^2 + 4 g_pq^2)), d = g_qq - g_pp
-              // (the textbook tau form without its division); c = 1 / sqrt(1 +
-              // t^2) keeps the rotation orthogonal to rounding for any t, so
-              // the reciprocals need not be correctly rounded
-              const double d = gqq - gpp;
-              const double h2 = d * d + 4.0 * a2;
-              const double den = fabs(d) + h2 * rsqrt_nr(h2);
-              tt = (d >= 0 ? 2.0 : -2.0) * apq * recip_nr(den);
-              c = rsqrt_nr(1.0 + tt * tt);
-              s = tt * c;
-              rotated[sweep] = 1;
-            }
-          }
-        } else {
-          p = 2 * t;
-          q = 2 * t + 1;
-        }
-        pp[t] = p;
-        qq[t] = q;
-        cs[t] = c;
-        sn[t] = s;
-        tn[t] = tt;
-      }
-      __syncthreads();
-      {
-        // G <- J^T G J on the 2 x 2 blocks (k, k2) of a full-square
-        // partition (each element has one owner; identity rotations store
-        // their values back unchanged), then R <- R J.  All loads of a thread
-        // precede its stores.
-        constexpr int KS = NT / HALF, MB = HALF / KS;
-        constexpr int RS = NT / NP, MR = HALF / RS;
-        static_assert(MB * KS == HALF && MR * RS == HALF, "block partition");
-        const int k = t % HALF, kb = t / HALF;
-        const int p = pp[k], q = qq[k];
-        const double c = cs[k], s = sn[k], tk = tn[k];
-        double g[MB][4], c2v[MB], s2v[MB];
-        int p2v[MB], q2v[MB];
-#pragma unroll
-        for (int i = 0; i < MB; ++i) {
-          const int k2 = kb + KS * i;
-          p2v[i] = pp[k2];
-          q2v[i] = qq[k2];
-          c2v[i] = cs[k2];
-          s2v[i] = sn[k2];
-          g[i][0] = G[p][p2v[i]];
-          g[i][1] = G[p][q2v[i]];
-          g[i][2] = G[q][p2v[i]];
-          g[i][3] = G[q][q2v[i]];
-        }
-        const int row = t % NP, rb = t / NP;
-        double ru[MR], rv[MR], rc[MR], rsn[MR];
-        int rp[MR], rq[MR];
-#pragma unroll
-        for (int i = 0; i < MR; ++i) {
-          const int kk = rb + RS * i;
-          rp[i] = pp[kk];
-          rq[i] = qq[kk];
-          rc[i] = cs[kk];
-          rsn[i] = sn[kk];
-          ru[i] = R[row][rp[i]];
-          rv[i] = R[row][rq[i]];
-        }
-#pragma unroll
-        for (int i = 0; i < MB; ++i) {
-          const double c2 = c2v[i], s2 = s2v[i];
-          const double t00 = c2 * g[i][0] - s2 * g[i][1], t01 = s2 * g[i][0] + c2 * g[i][1];
-          const double t10 = c2 * g[i][2] - s2 * g[i][3], t11 = s2 * g[i][2] + c2 * g[i][3];
-          double n00 = c * t00 - s * t10, n01 = c * t01 - s * t11;
-          double n10 = s * t00 + c * t10, n11 = s * t01 + c * t11;
-          // the rotated pair's own block: annihilated exactly, diagonal moved by t g_pq
-          const bool own = (kb + KS * i == k) && s != 0.0;
-          n00 = own ? g[i][0] - tk * g[i][1] : n00;
-          n11 = own ? g[i][3] + tk * g[i][1] : n11;
-          n01 = own ? 0.0 : n01;
-          n10 = own ? 0.0 : n10;
-          G[p][p2v[i]] = n00;
-          G[p][q2v[i]] = n01;
-          G[q][p2v[i]] = n10;
-          G[q][q2v[i]] = n11;
-        }
-        // (rows >= n hold zeros in the rotated columns)
-#pragma unroll
-        for (int i = 0; i < MR; ++i) {
-          R[row][rp[i]] = rc[i] * ru[i] - rsn[i] * rv[i];
-          R[row][rq[i]] = rsn[i] * ru[i] + rc[i] * rv[i];
+  // Rotation of slot k in round r (into parameter buffer b): the
+  // round-robin pair for k < npair, an identity on the spare indices
+  // (2k, 2k+1) >= m otherwise, so that the HALF slots partition all NP
+  // indices and every updating thread owns a fixed set of 2 x 2 blocks and R
+  // entries with no branches.
+  auto rotation = [&](int r, int k, int b, int sweep) {
+    int p, q;
+    double c = 1.0, s = 0.0, tt = 0.0;
+    if (k < npair) {
+      p = rr_p(r, k, m);
+      q = rr_q(r, k, m);
+      if (p < n && q < n) {
+        const double apq = G[p][q];
+        const double gpp = G[p][p], gqq = G[q][q];
+        const double app = fabs(gpp), aqq = fabs(gqq);
+        // skip test on squares (no square roots): |g_pq| against
+        // tol_in sqrt(g_pp g_qq) and the rounding floor 2 fl sqrt(max g)
+        const double a2 = apq * apq;
+        if (a2 > tol_in * tol_in * (app * aqq) && a2 > 4.0 * fl * fl * fmax(app, aqq) &&
+            fabs(apq) > 1e-300) {
+          // t = tan(theta) of the smaller rotation angle,
+          //   t = 2 g_pq sign(d) / (|d| + sqrt(d^2 + 4 g_pq^2)), d = g_qq - g_pp
+          // (the textbook tau form without its division); c = 1 / sqrt(1 +
+          // t^2) keeps the rotation orthogonal to rounding for any t, so
+          // the reciprocals need not be correctly rounded
+          const double d = gqq - gpp;
+          const double h2 = d * d + 4.0 * a2;
+          const double den = fabs(d) + h2 * rsqrt_nr(h2);
+          tt = (d >= 0 ? 2.0 : -2.0) * apq * recip_nr(den);
+          c = rsqrt_nr(1.0 + tt * tt);
+          s = tt * c;
+          rotated[sweep] = 1;
         }
       }
-      __syncthreads();
+    } else {
+      p = 2 * k;
+      q = 2 * k + 1;
     }
-    if (!rotated[sweep]) break;
+    pp[b][k] = p;
+    qq[b][k] = q;
+    cs[b][k] = c;
+    sn[b][k] = s;
+    tn[b][k] = tt;
+  };
+  // G <- J^T G J on the 2 x 2 blocks (k, k2) of a full-square partition
+  // (each element has one owner; identity rotations store their values back
+  // unchanged) by NTG threads, tg < NTG.  All loads of a thread precede its
+  // stores.
+  auto update_g = [&](int b, int tg) {
+    constexpr int KS = NTG / HALF, MB = HALF / KS;
+    static_assert(MB * KS == HALF, "block partition");
+    const int k = tg % HALF, kb = tg / HALF;
+    const int p = pp[b][k], q = qq[b][k];
+    const double c = cs[b][k], s = sn[b][k], tk = tn[b][k];
+    double g[MB][4], c2v[MB], s2v[MB];
+    int p2v[MB], q2v[MB];
+#pragma unroll
+    for (int i = 0; i < MB; ++i) {
+      const int k2 = kb + KS * i;
+      p2v[i] = pp[b][k2];
+      q2v[i] = qq[b][k2];
+      c2v[i] = cs[b][k2];
+      s2v[i] = sn[b][k2];
+      g[i][0] = G[p][p2v[i]];
+      g[i][1] = G[p][q2v[i]];
+      g[i][2] = G[q][p2v[i]];
+      g[i][3] = G[q][q2v[i]];
+    }
+#pragma unroll
+    for (int i = 0; i < MB; ++i) {
+      const double c2 = c2v[i], s2 = s2v[i];
+      const double t00 = c2 * g[i][0] - s2 * g[i][1], t01 = s2 * g[i][0] + c2 * g[i][1];
+      const double t10 = c2 * g[i][2] - s2 * g[i][3], t11 = s2 * g[i][2] + c2 * g[i][3];
+      double n00 = c * t00 - s * t10, n01 = c * t01 - s * t11;
+      double n10 = s * t00 + c * t10, n11 = s * t01 + c * t11;
+      // the rotated pair's own block: annihilated exactly, diagonal moved by t g_pq
+      const bool own = (kb + KS * i == k) && s != 0.0;
+      n00 = own ? g[i][0] - tk * g[i][1] : n00;
+      n11 = own ? g[i][3] + tk * g[i][1] : n11;
+      n01 = own ? 0.0 : n01;
+      n10 = own ? 0.0 : n10;
+      G[p][p2v[i]] = n00;
+      G[p][q2v[i]] = n01;
+      G[q][p2v[i]] = n10;
+      G[q][q2v[i]] = n11;
+    }
+  };
+  // R <- R J by NTR threads, tr < NTR, each on fixed rows and pair slots
+  // (rows >= n hold zeros in the rotated columns)
+  auto update_r = [&](int b, int tr) {
+    constexpr int RS = NTR / NP, MR = HALF / RS;
+    static_assert(MR * RS == HALF, "row partition");
+    const int row = tr % NP, rb = tr / NP;
+    double ru[MR], rv[MR], rc[MR], rsn[MR];
+    int rp[MR], rq[MR];
+#pragma unroll
+    for (int i = 0; i < MR; ++i) {
+      const int kk = rb + RS * i;
+      rp[i] = pp[b][kk];
+      rq[i] = qq[b][kk];
+      rc[i] = cs[b][kk];
+      rsn[i] = sn[b][kk];
+      ru[i] = R[row][rp[i]];
+      rv[i] = R[row][rq[i]];
+    }
+#pragma unroll
+    for (int i = 0; i < MR; ++i) {
+      R[row][rp[i]] = rc[i] * ru[i] - rsn[i] * rv[i];
+      R[row][rq[i]] = rsn[i] * ru[i] + rc[i] * rv[i];
+    }
+  };
+  if (!SPLIT) {
+    // one group: rotations, barrier, G and R updates, barrier
+    for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
+      for (int r = 0; r < nround; ++r) {
+        if (t < HALF) rotation(r, t, 0, sweep);
+        __syncthreads();
+        update_g(0, t);
+        update_r(0, t);
+        __syncthreads();
+      }
+      if (!rotated[sweep]) break;
+    }
+  } else {
+    // Two groups: threads below NTG update G; then, while wave 0 computes
+    // the next round's rotations from the updated G, the other NTR threads
+    // update R with this round's (double-buffered parameters).
+    if (t < HALF) rotation(0, t, 0, 0);
+    __syncthreads();
+    int b = 0;
+    for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
+      for (int r = 0; r < nround; ++r, b ^= 1) {
+        if (t < NTG) update_g(b, t);
+        __syncthreads();
+        const bool last = r + 1 == nround;
+        const int rn = last ? 0 : r + 1, sw = last ? sweep + 1 : sweep;
+        if (t < HALF && sw < inner_sweeps) rotation(rn, t, b ^ 1, sw);
+        if (t >= NTG) update_r(b, t - NTG);
+        __syncthreads();
+      }
+      if (!rotated[sweep]) break;
+    }
   }
 
   // ---- rotate this slice's rows: W[:, cols] <- W[:, cols] R, same for V ----
