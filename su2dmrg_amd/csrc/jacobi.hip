@@ -143,15 +143,17 @@ __global__ __launch_bounds__(NT) void gram_partial_kernel(const BlockDesc* __res
 // The pair's Gram matrix is diagonalised in LDS by parallel cyclic Jacobi
 // (round-robin rounds of disjoint rotations, inner_sweeps <= INNER_SWEEPS
 // sweeps: the outer iteration revisits the pair; one inner sweep per visit
-// costs a few more outer sweeps but the least time, since the round is
-// bound by its VALU instructions).  A round updates G <- J^T G J by 2 x 2 blocks, one
-// thread per block pair, with the rotated entry set to zero and the diagonal
+// costs a few more outer sweeps but the least time).  A round updates
+// G <- J^T G J by 2 x 2 blocks, with the rotated entry set to zero and the diagonal
 // moved by t g_pq; a rotation is skipped below max(tol_in sqrt(g_pp g_qq),
 // 0.2 tol_abs ||A||_F sqrt(max(g_pp, g_qq))), about ten times below the
 // outer test.  Round latency, not arithmetic, bounds this loop (a 32 x 32
 // round costs about as much as a 64 x 64 one), so the rotation parameters
-// avoid the correctly rounded square root and division sequences and the
-// 64 x 64 case runs 512 threads (two waves per SIMD).
+// avoid the correctly rounded square root and division sequences, and the
+// 64 x 64 case runs 512 threads (two waves per SIMD) in two groups: the
+// lower half updates G, then wave 0 computes the next round's rotations
+// while the upper half updates R with this round's (double-buffered)
+// parameters.
 template <int CB, int NT>
 __global__ __launch_bounds__(NT) void jacobi_rotate_kernel(const BlockDesc* __restrict__ blocks,
                                                            const PairJob* __restrict__ jobs,

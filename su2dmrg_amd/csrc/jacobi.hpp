@@ -13,7 +13,7 @@
 // Columns are cut into blocks of 16 (or 32).  A sweep visits every pair of blocks
 // once, in round-robin rounds of disjoint pairs; two kernel launches run all
 // pairs of a round for every block of the batch.  The first forms partial
-// Gram matrices (up to 32 x 32) of each pair over row slices of W, the second
+// Gram matrices (up to 64 x 64) of each pair over row slices of W, the second
 // sums them, diagonalises the pair's Gram matrix in LDS by parallel cyclic
 // Jacobi (one to three inner sweeps per visit) and applies the eigenvector
 // matrix to the pair's columns of W and V, each slice to its own rows.  A
@@ -47,7 +47,7 @@ struct JacobiBlock {
 // column of V has its first entry with |v| >= max|v|/2 made positive (the
 // LAPACK routine the reference calls leaves signs open), W's columns follow.
 // block_cols: 16 (pairs of 32 columns, one wave per pair) or 32 (pairs of 64
-// columns, four waves per pair); inner_sweeps: 1..3 inner Jacobi sweeps per
+// columns, eight waves per pair); inner_sweeps: 1..3 inner Jacobi sweeps per
 // visit of a pair.
 int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol = 1e-13,
                        int max_sweeps = 40, int block_cols = 32, int inner_sweeps = 1);
