@@ -34,14 +34,16 @@ def log(msg):
 def measured_traffic(variants):
     """HBM bytes of one matvec (stage 1 + stage 2 launches) from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary, when it was taken with the
-    same GEMM tile variants; None otherwise."""
+    same GEMM kernel and tile variants; None otherwise."""
     path = os.path.join(ROOT, "profiles", "heff_matvec_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    return t["bytes_per_matvec"] if list(t.get("gemm_variants", [])) == list(variants) else None
+    same = (list(t.get("gemm_variants", [])) == list(variants)
+            and t.get("kernel") == "grouped_gemm_f64_uniform")
+    return t["bytes_per_matvec"] if same else None
 
 
 def cpu_baseline(work, budget_s=20.0):
@@ -227,7 +229,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
             "traffic": measured_traffic(chosen),
-            "kernel": "grouped_gemm_f64_kernel (stage 1 + stage 2 launches of one matvec; "
+            "kernel": "grouped_gemm_f64_uniform (stage 1 + stage 2 launches of one matvec; "
                       "traffic in bytes per matvec from profiles/heff_matvec_traffic.json)",
         },
         "cpu_baseline": cpu,
