@@ -1,5 +1,8 @@
 #include "blas1.hpp"
 
+#include <algorithm>
+#include <cstring>
+
 #include "common.hpp"
 #include "context.hpp"
 
@@ -73,21 +76,26 @@ __global__ __launch_bounds__(kThreads) void dot_final_kernel(double* __restrict_
   if (threadIdx.x == 0) part[kDotBlocks] = sh[0];
 }
 
-__global__ void strided_axpy_kernel(const StridedAxpy* __restrict__ desc, int ndesc,
-                                    long long total) {
-  long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= total) return;
-  int lo = 0, hi = ndesc - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (desc[mid].elem_begin <= e) lo = mid;
-    else hi = mid - 1;
+constexpr int kAxpyChunk = 4096;   // elements of one descriptor per workgroup
+
+struct AxpyChunk {
+  int desc, first, count, pad;
+};
+
+// One workgroup per chunk: elements [first, first + count) of one descriptor
+// in column-major order (r fastest), so the source reads coalesce.
+__global__ __launch_bounds__(kThreads) void strided_axpy_kernel(
+    const StridedAxpy* __restrict__ desc, const AxpyChunk* __restrict__ chunks) {
+  const AxpyChunk ch = chunks[blockIdx.x];
+  const StridedAxpy d = desc[ch.desc];
+  const int rows = d.rows;
+  for (int j = threadIdx.x; j < ch.count; j += kThreads) {
+    const int i = ch.first + j;
+    const int c = i / rows, r = i - c * rows;
+    // a null source adds alpha (an identity block when rs/cs walk a diagonal)
+    double* p = d.dst + (r * d.rs + c * d.cs);
+    *p += d.src ? d.alpha * d.src[r + c * d.ld] : d.alpha;
   }
-  const StridedAxpy d = desc[lo];
-  long long i = e - d.elem_begin;
-  long long r = i % d.rows, c = i / d.rows;
-  // a null source adds alpha (an identity block when rs/cs walk a diagonal)
-  d.dst[r * d.rs + c * d.cs] += d.src ? d.alpha * d.src[r + c * d.ld] : d.alpha;
 }
 
 unsigned grid_for(long long n) {
@@ -144,12 +152,47 @@ double dev_dot(Context& c, const double* x, const double* y, long long n) {
   return h[0];
 }
 
-void dev_strided_axpy(Context& c, const StridedAxpy* d_desc, int ndesc, long long total) {
-  if (total <= 0 || ndesc <= 0) return;
-  long long blocks = (total + kThreads - 1) / kThreads;
-  hipLaunchKernelGGL(strided_axpy_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, c.stream(),
-                     d_desc, ndesc, total);
-  SU2_HIP(hipGetLastError());
+void dev_strided_axpy_launches(Context& c,
+                               const std::vector<std::vector<StridedAxpy>>& launches) {
+  std::vector<StridedAxpy> flat;
+  std::vector<AxpyChunk> chunks;
+  std::vector<std::pair<size_t, size_t>> span;   // chunk range of each launch
+  for (auto& l : launches) {
+    const size_t c0 = chunks.size();
+    for (const StridedAxpy& d : l) {
+      const long long n = (long long)d.rows * d.cols;
+      if (n <= 0) continue;
+      SU2_CHECK(n < (1LL << 31), kErrCapacity, "strided axpy: block of %lld elements", n);
+      const int di = (int)flat.size();
+      flat.push_back(d);
+      for (long long f = 0; f < n; f += kAxpyChunk)
+        chunks.push_back(AxpyChunk{di, (int)f, (int)std::min<long long>(kAxpyChunk, n - f), 0});
+    }
+    span.push_back({c0, chunks.size()});
+  }
+  if (chunks.empty()) return;
+  const size_t dbytes = (flat.size() * sizeof(StridedAxpy) + 15) / 16 * 16;
+  const size_t bytes = dbytes + chunks.size() * sizeof(AxpyChunk);
+  std::vector<char> host(bytes);
+  std::memcpy(host.data(), flat.data(), flat.size() * sizeof(StridedAxpy));
+  std::memcpy(host.data() + dbytes, chunks.data(), chunks.size() * sizeof(AxpyChunk));
+  char* dev = nullptr;
+  SU2_HIP(hipMalloc(&dev, bytes));
+  SU2_HIP(hipMemcpyAsync(dev, host.data(), bytes, hipMemcpyHostToDevice, c.stream()));
+  const StridedAxpy* d_desc = (const StridedAxpy*)dev;
+  const AxpyChunk* d_chunks = (const AxpyChunk*)(dev + dbytes);
+  for (auto& s : span) {
+    if (s.second == s.first) continue;
+    hipLaunchKernelGGL(strided_axpy_kernel, dim3((unsigned)(s.second - s.first)), dim3(kThreads),
+                       0, c.stream(), d_desc, d_chunks + s.first);
+    SU2_HIP(hipGetLastError());
+  }
+  SU2_HIP(hipStreamSynchronize(c.stream()));
+  SU2_HIP(hipFree(dev));
+}
+
+void dev_strided_axpy(Context& c, const std::vector<StridedAxpy>& desc) {
+  dev_strided_axpy_launches(c, {desc});
 }
 
 }  // namespace su2

@@ -8,6 +8,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 namespace su2 {
 
 class Context;
@@ -28,10 +30,15 @@ struct StridedAxpy {
   double* dst;
   const double* src;
   long long rs, cs, ld;
-  int rows, cols;
+  int rows, cols;   // rows * cols < 2^31
   double alpha;
-  long long elem_begin;
 };
-void dev_strided_axpy(Context& c, const StridedAxpy* d_desc, int ndesc, long long total);
+// One launch per inner vector, in order (a later launch may add into what an
+// earlier one wrote); the descriptors of all launches are uploaded together
+// and the call returns when the last launch has finished.  Every descriptor
+// is cut into chunks of up to 4096 elements, one workgroup each, so a
+// workgroup reads its descriptor once and indexes with 32-bit arithmetic.
+void dev_strided_axpy_launches(Context& c, const std::vector<std::vector<StridedAxpy>>& launches);
+void dev_strided_axpy(Context& c, const std::vector<StridedAxpy>& desc);
 
 }  // namespace su2

@@ -156,12 +156,13 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
   dev_fill(c, d_work_, work, 0.0);
 
   // bundle assembly: one batched strided-axpy launch per summand position, so
-  // coupling sums accumulate in the order given (deterministic)
+  // coupling sums accumulate in the order given (deterministic); then H_L of
+  // each sector transposed (source row r lands in column r)
+  std::vector<std::vector<StridedAxpy>> launches;
   size_t maxside = 0;
   for (auto& t : terms) maxside = std::max({maxside, t.left.size(), t.right.size()});
   for (size_t pos = 0; pos < maxside; ++pos) {
     std::vector<StridedAxpy> desc;
-    long long total = 0;
     for (int ti = 0; ti < T; ++ti) {
       const HeffTerm& t = terms[ti];
       const int tl = local[ti];
@@ -180,8 +181,6 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
             d.rows = tr.Da;
             d.cols = tr.Dap;
             d.alpha = t.left[pos].first;
-            d.elem_begin = total;
-            total += (long long)tr.Da * tr.Dap;
             desc.push_back(d);
           }
         }
@@ -198,50 +197,32 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
             d.rows = tr.Db;
             d.cols = tr.Dbp;
             d.alpha = t.right[pos].first;
-            d.elem_begin = total;
-            total += (long long)tr.Db * tr.Dbp;
             desc.push_back(d);
           }
         }
       }
     }
-    if (desc.empty()) continue;
-    StridedAxpy* d_desc = nullptr;
-    SU2_HIP(hipMalloc(&d_desc, desc.size() * sizeof(StridedAxpy)));
-    SU2_HIP(hipMemcpyAsync(d_desc, desc.data(), desc.size() * sizeof(StridedAxpy),
-                           hipMemcpyHostToDevice, c.stream()));
-    dev_strided_axpy(c, d_desc, (int)desc.size(), total);
-    SU2_HIP(hipStreamSynchronize(c.stream()));
-    SU2_HIP(hipFree(d_desc));
+    if (!desc.empty()) launches.push_back(std::move(desc));
   }
-
   if (!hlt_off.empty()) {
     std::vector<StridedAxpy> desc;
-    long long total = 0;
     for (auto& kv : hlt_off) {
       const Block& hb = hl->blocks()[hl->find2(kv.first, kv.first)];
       const int D = hb.dim[0];
       StridedAxpy d{};
       d.dst = d_work_ + kv.second;
       d.src = hl->data() + hb.offset;
-      d.rs = D;  // source row r lands in column r
+      d.rs = D;
       d.cs = 1;
       d.ld = D;
       d.rows = D;
       d.cols = D;
       d.alpha = 1.0;
-      d.elem_begin = total;
-      total += (long long)D * D;
       desc.push_back(d);
     }
-    StridedAxpy* d_desc = nullptr;
-    SU2_HIP(hipMalloc(&d_desc, desc.size() * sizeof(StridedAxpy)));
-    SU2_HIP(hipMemcpyAsync(d_desc, desc.data(), desc.size() * sizeof(StridedAxpy),
-                           hipMemcpyHostToDevice, c.stream()));
-    dev_strided_axpy(c, d_desc, (int)desc.size(), total);
-    SU2_HIP(hipStreamSynchronize(c.stream()));
-    SU2_HIP(hipFree(d_desc));
+    launches.push_back(std::move(desc));
   }
+  dev_strided_axpy_launches(c, launches);
 
   // stage 1: X(j->j') = f * Lb^T psi(j)   [T*D_a(j') x D_b(j)]
   for (auto& tr : trs) {

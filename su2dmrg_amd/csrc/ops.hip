@@ -49,34 +49,7 @@ void ScatterBatch::add(const void* key, const StridedAxpy& d) {
   waves_[w].push_back(d);
 }
 
-void ScatterBatch::run(Context& c) {
-  size_t total = 0;
-  for (auto& w : waves_) total += w.size();
-  if (total == 0) return;
-  StridedAxpy* d = nullptr;
-  SU2_HIP(hipMalloc(&d, total * sizeof(StridedAxpy)));
-  size_t off = 0;
-  std::vector<std::pair<size_t, long long>> launch;
-  std::vector<StridedAxpy> flat;
-  flat.reserve(total);
-  for (auto& w : waves_) {
-    long long elems = 0;
-    for (auto x : w) {
-      x.elem_begin = elems;
-      elems += (long long)x.rows * x.cols;
-      flat.push_back(x);
-    }
-    launch.push_back({w.size(), elems});
-  }
-  SU2_HIP(hipMemcpyAsync(d, flat.data(), total * sizeof(StridedAxpy), hipMemcpyHostToDevice,
-                         c.stream()));
-  for (auto& l : launch) {
-    dev_strided_axpy(c, d + off, (int)l.first, l.second);
-    off += l.first;
-  }
-  SU2_HIP(hipStreamSynchronize(c.stream()));
-  SU2_HIP(hipFree(d));
-}
+void ScatterBatch::run(Context& c) { dev_strided_axpy_launches(c, waves_); }
 
 void dev_scale_columns(Context& c, double* x, int m, int n, const double* d_f) {
   const long long tot = (long long)m * n;
