@@ -176,8 +176,7 @@ void dev_strided_axpy_launches(Context& c,
   std::vector<char> host(bytes);
   std::memcpy(host.data(), flat.data(), flat.size() * sizeof(StridedAxpy));
   std::memcpy(host.data() + dbytes, chunks.data(), chunks.size() * sizeof(AxpyChunk));
-  char* dev = nullptr;
-  SU2_HIP(hipMalloc(&dev, bytes));
+  char* dev = (char*)c.dev_alloc(bytes);
   SU2_HIP(hipMemcpyAsync(dev, host.data(), bytes, hipMemcpyHostToDevice, c.stream()));
   const StridedAxpy* d_desc = (const StridedAxpy*)dev;
   const AxpyChunk* d_chunks = (const AxpyChunk*)(dev + dbytes);
@@ -187,8 +186,7 @@ void dev_strided_axpy_launches(Context& c,
                        0, c.stream(), d_desc, d_chunks + s.first);
     SU2_HIP(hipGetLastError());
   }
-  SU2_HIP(hipStreamSynchronize(c.stream()));
-  SU2_HIP(hipFree(dev));
+  c.dev_free(dev);
 }
 
 void dev_strided_axpy(Context& c, const std::vector<StridedAxpy>& desc) {

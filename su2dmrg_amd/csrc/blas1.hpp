@@ -33,9 +33,9 @@ struct StridedAxpy {
   int rows, cols;   // rows * cols < 2^31
   double alpha;
 };
-// One launch per inner vector, in order (a later launch may add into what an
-// earlier one wrote); the descriptors of all launches are uploaded together
-// and the call returns when the last launch has finished.  Every descriptor
+// One launch per inner vector, in order on the context stream (a later
+// launch may add into what an earlier one wrote); the descriptors of all
+// launches are uploaded together and the call does not wait.  Every descriptor
 // is cut into chunks of up to 4096 elements, one workgroup each, so a
 // workgroup reads its descriptor once and indexes with 32-bit arithmetic.
 void dev_strided_axpy_launches(Context& c, const std::vector<std::vector<StridedAxpy>>& launches);

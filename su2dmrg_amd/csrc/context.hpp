@@ -8,10 +8,44 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <memory>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 namespace su2 {
+
+// Device memory cache of one context's stream.  A freed block is held back
+// until the stream has been synchronised after the free (so no kernel or
+// copy still uses it), then serves a request of at least half its size.  A
+// request that only a held-back block would fit synchronises the stream;
+// otherwise new memory comes from hipMalloc.  Cached blocks return to the
+// driver when the pool is destroyed or hipMalloc runs out of memory.
+// Objects that may outlive their context (tensors, GEMM and H_eff plans)
+// share ownership of the pool: once the context is gone (detach), the pool
+// hands back what it holds and frees later blocks with hipFree directly.
+class DevicePool {
+ public:
+  DevicePool(int device, hipStream_t stream) : device_(device), stream_(stream) {}
+  ~DevicePool();
+  DevicePool(const DevicePool&) = delete;
+  DevicePool& operator=(const DevicePool&) = delete;
+  void* alloc(size_t bytes);   // alloc(0) is nullptr
+  void free(void* p);
+  void set_stream(hipStream_t s);
+  void detach();
+
+ private:
+  void recycle();        // synchronise the stream, held-back blocks become free
+  void release_free();   // free blocks back to the driver
+  int device_;
+  hipStream_t stream_;
+  bool detached_ = false;
+  std::multimap<size_t, void*> free_;           // size -> block, reusable now
+  std::vector<std::pair<size_t, void*>> held_;  // freed since the last sync
+  std::unordered_map<void*, size_t> live_;      // handed out
+};
 
 class Context {
  public:
@@ -22,8 +56,17 @@ class Context {
 
   int device() const { return device_; }
   hipStream_t stream() const { return stream_; }
-  void set_stream(hipStream_t s) { stream_ = s; }
+  void set_stream(hipStream_t s);
   void make_current() const;
+
+  // Device memory for work on this context's stream (DevicePool).
+  void* dev_alloc(size_t bytes) { return pool_->alloc(bytes); }
+  void dev_free(void* p) { pool_->free(p); }
+  template <class T>
+  T* dev_alloc_n(size_t n) {
+    return (T*)dev_alloc(n * sizeof(T));
+  }
+  const std::shared_ptr<DevicePool>& pool() const { return pool_; }
 
   std::string last_error;
 
@@ -39,6 +82,7 @@ class Context {
  private:
   int device_;
   hipStream_t stream_ = nullptr;
+  std::shared_ptr<DevicePool> pool_;
   bool own_stream_ = false;
   double* d_red_ = nullptr;
   size_t d_red_n_ = 0;

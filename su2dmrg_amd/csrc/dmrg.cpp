@@ -299,8 +299,7 @@ std::unique_ptr<Env> Dmrg::project(const Tensor& P, const Env& hat) {
       jobs.push_back(j);
     }
   }
-  double* Y = nullptr;
-  if (ytot > 0) SU2_HIP(hipMalloc(&Y, ytot * sizeof(double)));
+  double* Y = c.dev_alloc_n<double>(ytot > 0 ? ytot : 0);
   {
     GemmBatch g;
     for (auto& j : jobs) g.add(Y + j.y, j.Da, j.Da, j.db, j.O, j.Da, 0, j.Pb, j.Db, 0, j.Db);
@@ -311,8 +310,7 @@ std::unique_ptr<Env> Dmrg::project(const Tensor& P, const Env& hat) {
     for (auto& j : jobs) g.add(j.dst, j.da, j.da, j.db, j.Pa, j.Da, 1, Y + j.y, j.Da, 0, j.Da);
     g.run(c);
   }
-  SU2_HIP(hipStreamSynchronize(c.stream()));
-  if (Y) SU2_HIP(hipFree(Y));
+  c.dev_free(Y);
   stats_.t_project += now() - t0;
   return out;
 }
@@ -444,7 +442,7 @@ std::unique_ptr<Tensor> Dmrg::start_left(int nl, int nr, const Tensor& A, const 
     if (ob < 0 || pb < 0 || F.dim(jl) == 0) continue;
     const long long Df = F.dim(jl);
     double* blk = nullptr;
-    SU2_HIP(hipMalloc(&blk, (size_t)dl * Df * sizeof(double)));
+    blk = c.dev_alloc_n<double>((size_t)dl * Df);
     dev_fill(c, blk, (long long)dl * Df, 0.0);
     tmp.push_back(blk);
     for (auto& p : F.parts(jl)) {
@@ -469,8 +467,7 @@ std::unique_ptr<Tensor> Dmrg::start_left(int nl, int nr, const Tensor& A, const 
   }
   sb.run(c);
   g.run(c);
-  SU2_HIP(hipStreamSynchronize(c.stream()));
-  for (double* p : tmp) SU2_HIP(hipFree(p));
+  for (double* p : tmp) c.dev_free(p);
   stats_.t_start += now() - t0;
   return out;
 }
@@ -497,7 +494,7 @@ std::unique_ptr<Tensor> Dmrg::start_right(int nl, int nr, const Tensor& B, const
     if (ob < 0 || pb < 0 || E.dim(jr) == 0) continue;
     const long long De = E.dim(jr);
     double* blk = nullptr;
-    SU2_HIP(hipMalloc(&blk, (size_t)De * dr * sizeof(double)));
+    blk = c.dev_alloc_n<double>((size_t)De * dr);
     dev_fill(c, blk, De * dr, 0.0);
     tmp.push_back(blk);
     for (auto& p : E.parts(jr)) {
@@ -522,8 +519,7 @@ std::unique_ptr<Tensor> Dmrg::start_right(int nl, int nr, const Tensor& B, const
   }
   sb.run(c);
   g.run(c);
-  SU2_HIP(hipStreamSynchronize(c.stream()));
-  for (double* p : tmp) SU2_HIP(hipFree(p));
+  for (double* p : tmp) c.dev_free(p);
   stats_.t_start += now() - t0;
   return out;
 }

@@ -38,14 +38,12 @@ Leg with_dir(Leg l, int d) {
   return l;
 }
 
+// device scratch from the context's cache, returned on scope exit
 struct DevScratch {
-  double* p = nullptr;
-  explicit DevScratch(long long n) {
-    if (n > 0) SU2_HIP(hipMalloc(&p, n * sizeof(double)));
-  }
-  ~DevScratch() {
-    if (p) (void)hipFree(p);
-  }
+  Context& c;
+  double* p;
+  DevScratch(Context& ctx, long long n) : c(ctx), p(ctx.dev_alloc_n<double>(n > 0 ? n : 0)) {}
+  ~DevScratch() { c.dev_free(p); }
   DevScratch(const DevScratch&) = delete;
   DevScratch& operator=(const DevScratch&) = delete;
 };
@@ -149,7 +147,7 @@ std::unique_ptr<Tensor> transfer(Context& c, const Tensor& prev, const Tensor& P
     ytot += (long long)Pl.blocks()[lb].dim[0] * Pc.blocks()[cb].dim[1];
   }
   if (out->size()) dev_fill(c, out->data(), out->size(), 0.0);
-  DevScratch Y(ytot);
+  DevScratch Y(c, ytot);
   if (ytot) dev_fill(c, Y.p, ytot, 0.0);
   GemmBatch g1, g2;
   for (const Job& j : jobs) {
@@ -297,7 +295,7 @@ std::vector<std::unique_ptr<Tensor>> Dmrg::lower_vectors(int il, int ir, const F
       jobs.push_back({b, ab, bb, tot, tot + (long long)PB.dim[0] * dlow});
       tot += (long long)(PB.dim[0] + PB.dim[1]) * dlow;
     }
-    DevScratch X(tot);
+    DevScratch X(c, tot);
     if (tot) dev_fill(c, X.p, tot, 0.0);
     GemmBatch g1, g2;
     for (const Job& j : jobs) {

@@ -17,6 +17,7 @@
 #include <cmath>
 
 #include "common.hpp"
+#include "context.hpp"
 
 namespace su2 {
 
@@ -656,10 +657,12 @@ int gemm_num_variants() { return kNumVariants; }
 GroupedGemm::~GroupedGemm() { release(); }
 
 void GroupedGemm::release() {
-  if (d_probs_) (void)hipFree(d_probs_);
-  if (d_slices_) (void)hipFree(d_slices_);
-  if (d_reds_) (void)hipFree(d_reds_);
-  if (d_ws_) (void)hipFree(d_ws_);
+  if (pool_) {
+    pool_->free(d_probs_);
+    pool_->free(d_slices_);
+    pool_->free(d_reds_);
+    pool_->free(d_ws_);
+  }
   d_probs_ = nullptr;
   d_slices_ = nullptr;
   d_reds_ = nullptr;
@@ -713,8 +716,9 @@ void GroupedGemm::add_slice(int problem, const double* A, long long lda, int tra
   flops_ += 2.0 * p.M * (double)p.N * K;
 }
 
-void GroupedGemm::finalize(int target_units, int variant) {
+void GroupedGemm::finalize(Context& c, int target_units, int variant) {
   SU2_CHECK(variant_ < 0, kErrArgument, "gemm plan already finalized");
+  pool_ = c.pool();
   for (auto& p : probs_)
     SU2_CHECK(p.slice_end > p.slice_begin, kErrArgument, "gemm: problem without K slices");
   target_units_ = std::max(1, target_units);
@@ -782,20 +786,20 @@ void GroupedGemm::set_variant(int v) {
   total_units_ = units;
   SU2_CHECK(total_units_ < (1ll << 31), kErrCapacity, "gemm: too many work units");
   if (!run_probs_.empty()) {
-    SU2_HIP(hipMalloc(&d_probs_, run_probs_.size() * sizeof(GemmProblem)));
+    d_probs_ = (GemmProblem*)pool_->alloc(run_probs_.size() * sizeof(GemmProblem));
     SU2_HIP(hipMemcpy(d_probs_, run_probs_.data(), run_probs_.size() * sizeof(GemmProblem),
                       hipMemcpyHostToDevice));
   }
   if (!sl.empty()) {
-    SU2_HIP(hipMalloc(&d_slices_, sl.size() * sizeof(GemmSlice)));
+    d_slices_ = (GemmSlice*)pool_->alloc(sl.size() * sizeof(GemmSlice));
     SU2_HIP(hipMemcpy(d_slices_, sl.data(), sl.size() * sizeof(GemmSlice),
                       hipMemcpyHostToDevice));
   }
   if (!reds_.empty()) {
-    SU2_HIP(hipMalloc(&d_reds_, reds_.size() * sizeof(ReduceProblem)));
+    d_reds_ = (ReduceProblem*)pool_->alloc(reds_.size() * sizeof(ReduceProblem));
     SU2_HIP(hipMemcpy(d_reds_, reds_.data(), reds_.size() * sizeof(ReduceProblem),
                       hipMemcpyHostToDevice));
-    SU2_HIP(hipMalloc(&d_ws_, ws * sizeof(double)));
+    d_ws_ = (double*)pool_->alloc(ws * sizeof(double));
   }
 }
 

@@ -12,9 +12,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <vector>
 
 namespace su2 {
+
+class Context;
+class DevicePool;
 
 struct GemmSlice {
   const double* A;
@@ -64,8 +68,9 @@ class GroupedGemm {
   void add_slice(int problem, const double* A, long long lda, int transA,
                  const double* B, long long ldb, int transB, int K, double alpha);
   // Chooses K splits for about target_units work units with tile variant
-  // `variant` (-1: default) and uploads the tables.
-  void finalize(int target_units, int variant = -1);
+  // `variant` (-1: default) and uploads the tables, in device memory of `c`
+  // (the plan then launches on any stream but must not outlive c).
+  void finalize(Context& c, int target_units, int variant = -1);
   // Re-tiles for another variant (device tables rebuilt).
   void set_variant(int v);
   int variant() const { return variant_; }
@@ -98,6 +103,7 @@ class GroupedGemm {
   long long total_red_elems_ = 0;
   double flops_ = 0;
   int target_units_ = 2048;
+  std::shared_ptr<DevicePool> pool_;   // device memory of the finalizing context
   int variant_ = -1;
   int layout_ = -1;
   bool uniform_ = true;

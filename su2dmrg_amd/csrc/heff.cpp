@@ -37,7 +37,7 @@ void check_scalar(const Tensor& h, const Leg& l, const char* what) {
 
 HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor* hr,
                    const std::vector<HeffTerm>& terms)
-    : ctx_(c) {
+    : ctx_(c), pool_(c.pool()) {
   c.make_current();
   SU2_CHECK(psi.nleg() == 2 && psi.locspin() == 0, kErrStructure,
             "H_eff wavefunction must be a two-leg singlet tensor");
@@ -150,9 +150,9 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
         work += (long long)b.dim[0] * b.dim[0];
       }
     }
-  SU2_HIP(hipMalloc(&d_in_, std::max<long long>(psi_size_, 1) * sizeof(double)));
-  SU2_HIP(hipMalloc(&d_out_, std::max<long long>(psi_size_, 1) * sizeof(double)));
-  if (work > 0) SU2_HIP(hipMalloc(&d_work_, work * sizeof(double)));
+  d_in_ = c.dev_alloc_n<double>(std::max<long long>(psi_size_, 1));
+  d_out_ = c.dev_alloc_n<double>(std::max<long long>(psi_size_, 1));
+  if (work > 0) d_work_ = c.dev_alloc_n<double>(work);
   dev_fill(c, d_work_, work, 0.0);
 
   // bundle assembly: one batched strided-axpy launch per summand position, so
@@ -262,15 +262,15 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
     }
     if (!any) zero_blocks_.push_back({b.offset, b.size});
   }
-  stage1_.finalize(kTargetUnits);
-  stage2_.finalize(kTargetUnits);
+  stage1_.finalize(c, kTargetUnits);
+  stage2_.finalize(c, kTargetUnits);
   SU2_HIP(hipStreamSynchronize(c.stream()));
 }
 
 HeffPlan::~HeffPlan() {
-  if (d_in_) (void)hipFree(d_in_);
-  if (d_out_) (void)hipFree(d_out_);
-  if (d_work_) (void)hipFree(d_work_);
+  pool_->free(d_in_);
+  pool_->free(d_out_);
+  pool_->free(d_work_);
 }
 
 void HeffPlan::apply_raw(const double* psi, double* out) {

@@ -65,6 +65,7 @@ class HeffPlan {
 
  private:
   Context& ctx_;
+  std::shared_ptr<DevicePool> pool_;   // frees the buffers even after ctx_ is gone
   long long psi_size_ = 0;
   double* d_in_ = nullptr;    // plan-owned copy of the input vector
   double* d_out_ = nullptr;   // plan-owned output vector

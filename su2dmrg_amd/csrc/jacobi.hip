@@ -569,14 +569,12 @@ __global__ __launch_bounds__(256) void col_sign_kernel(const double* __restrict_
 
 unsigned nblk(long long n, int t) { return (unsigned)std::max<long long>(1, (n + t - 1) / t); }
 
+// device memory from the context's cache, returned on scope exit
 struct DevMem {
+  Context& c;
   void* p = nullptr;
-  explicit DevMem(size_t bytes) {
-    if (bytes) SU2_HIP(hipMalloc(&p, bytes));
-  }
-  ~DevMem() {
-    if (p) (void)hipFree(p);
-  }
+  DevMem(Context& ctx, size_t bytes) : c(ctx), p(ctx.dev_alloc(bytes)) {}
+  ~DevMem() { c.dev_free(p); }
   DevMem(const DevMem&) = delete;
   DevMem& operator=(const DevMem&) = delete;
 };
@@ -610,8 +608,8 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
   std::vector<double*> Wc(nb), Vc(nb), Wo(nb), Vo(nb);
   for (int b = 0; b < nb; ++b) {
     JacobiBlock& J = blocks[b];
-    W2[b].reset(new DevMem(sizeof(double) * (size_t)L[b] * K[b]));
-    V2[b].reset(new DevMem(sizeof(double) * (size_t)K[b] * K[b]));
+    W2[b].reset(new DevMem(c, sizeof(double) * (size_t)L[b] * K[b]));
+    V2[b].reset(new DevMem(c, sizeof(double) * (size_t)K[b] * K[b]));
     Wc[b] = J.W;
     Vc[b] = J.V;
     Wo[b] = (double*)W2[b]->p;
@@ -622,8 +620,8 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
                        K[b]);
   }
   SU2_HIP(hipGetLastError());
-  DevMem dn(sizeof(double) * noff[nb]), dperm(sizeof(int) * noff[nb]),
-      done(sizeof(double) * noff[nb]);
+  DevMem dn(c, sizeof(double) * noff[nb]), dperm(c, sizeof(int) * noff[nb]),
+      done(c, sizeof(double) * noff[nb]);
   std::vector<double> hn(noff[nb]);
   std::vector<int> hp(noff[nb]);
   {
@@ -686,9 +684,9 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
     all.insert(all.end(), rounds[r].begin(), rounds[r].end());
   }
   round_off[rounds.size()] = (int)all.size();
-  DevMem dgpart(sizeof(double) * NP * NP * std::max<size_t>(1, gpart_max));
-  DevMem ddesc(sizeof(BlockDesc) * nb), djobs(sizeof(PairJob) * std::max<size_t>(1, all.size())),
-      doff(sizeof(unsigned long long) * nb);
+  DevMem dgpart(c, sizeof(double) * NP * NP * std::max<size_t>(1, gpart_max));
+  DevMem ddesc(c, sizeof(BlockDesc) * nb), djobs(c, sizeof(PairJob) * std::max<size_t>(1, all.size())),
+      doff(c, sizeof(unsigned long long) * nb);
   if (!all.empty())
     SU2_HIP(hipMemcpyAsync(djobs.p, all.data(), sizeof(PairJob) * all.size(),
                            hipMemcpyHostToDevice, st));
@@ -774,7 +772,7 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
     std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return w2[x] < w2[y]; });
     J.s.resize(k);
     for (int i = 0; i < k; ++i) J.s[i] = std::sqrt(w2[perm[i]]);
-    DevMem dpb(sizeof(int) * k), dsign(sizeof(double) * k), did(sizeof(int) * k);
+    DevMem dpb(c, sizeof(int) * k), dsign(c, sizeof(double) * k), did(c, sizeof(int) * k);
     std::vector<int> ident(k);
     std::iota(ident.begin(), ident.end(), 0);
     SU2_HIP(hipMemcpyAsync(dpb.p, perm.data(), sizeof(int) * k, hipMemcpyHostToDevice, st));

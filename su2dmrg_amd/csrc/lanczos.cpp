@@ -72,8 +72,8 @@ LanczosResult lanczos_ground(Context& c, const std::function<void(const double*,
   LanczosResult res;
   std::vector<double*> ff(mlanc + 1, nullptr);
   double* tmp = nullptr;
-  for (auto& p : ff) SU2_HIP(hipMalloc(&p, std::max<long long>(n, 1) * sizeof(double)));
-  SU2_HIP(hipMalloc(&tmp, std::max<long long>(n, 1) * sizeof(double)));
+  for (auto& p : ff) p = c.dev_alloc_n<double>(std::max<long long>(n, 1));
+  tmp = c.dev_alloc_n<double>(std::max<long long>(n, 1));
   std::vector<double> aal(mlanc, 0.0), nnl(mlanc + 1, 0.0);
   auto hv = [&](const double* x, double* y) {
     apply(x, y);
@@ -99,9 +99,8 @@ LanczosResult lanczos_ground(Context& c, const std::function<void(const double*,
     return std::fabs(dev_dot(c, tmp, psi, n) / prod);
   };
   auto finish = [&]() {
-    SU2_HIP(hipStreamSynchronize(c.stream()));
-    for (auto p : ff) (void)hipFree(p);
-    (void)hipFree(tmp);
+    for (auto p : ff) c.dev_free(p);
+    c.dev_free(tmp);
   };
 
   // ff[0] = normalised start vector (initialize_lanczos, lanczos_su2.cpp:36-47)

@@ -14,21 +14,22 @@ namespace su2 {
 
 namespace {
 
+// device buffer from the context's cache, returned when destroyed
 struct DevBuf {
+  Context* c = nullptr;
   double* p = nullptr;
   DevBuf() = default;
-  explicit DevBuf(long long n) {
-    if (n > 0) SU2_HIP(hipMalloc(&p, n * sizeof(double)));
-  }
+  DevBuf(Context& ctx, long long n) : c(&ctx), p(ctx.dev_alloc_n<double>(n > 0 ? n : 0)) {}
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (c) c->dev_free(p);
   }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p) { o.p = nullptr; }
+  DevBuf(DevBuf&& o) noexcept : c(o.c), p(o.p) { o.p = nullptr; }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
-      if (p) (void)hipFree(p);
+      if (c) c->dev_free(p);
+      c = o.c;
       p = o.p;
       o.p = nullptr;
     }
@@ -77,8 +78,8 @@ SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int
       Sec& s = secs[i];
       if (s.k == 0) continue;
       const int L = std::max(s.m, s.n);
-      s.U = DevBuf((long long)L * s.k);
-      s.V = DevBuf((long long)s.k * s.k);
+      s.U = DevBuf(c, (long long)L * s.k);
+      s.V = DevBuf(c, (long long)s.k * s.k);
       JacobiBlock b;
       b.A = s.A;
       b.m = s.m;

@@ -14,16 +14,107 @@ Context::Context(int device) : device_(device) {
   // default priority stream (never a prioritised one)
   SU2_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   own_stream_ = true;
+  pool_ = std::make_shared<DevicePool>(device, stream_);
 }
 
 Context::~Context() {
   (void)hipSetDevice(device_);
+  pool_->detach();
   if (d_red_) (void)hipFree(d_red_);
   if (h_red_) (void)hipHostFree(h_red_);
   if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
 }
 
 void Context::make_current() const { SU2_HIP(hipSetDevice(device_)); }
+
+void Context::set_stream(hipStream_t s) {
+  pool_->set_stream(s);
+  stream_ = s;
+}
+
+namespace {
+// allocation granule: 256 B up to 1 MiB, 64 KiB up to 64 MiB, 2 MiB above
+size_t pool_round(size_t b) {
+  const size_t g = b <= (size_t(1) << 20) ? 256 : b <= (size_t(64) << 20) ? (size_t(64) << 10)
+                                                                         : (size_t(2) << 20);
+  return (b + g - 1) / g * g;
+}
+}  // namespace
+
+DevicePool::~DevicePool() {
+  (void)hipSetDevice(device_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto& x : held_) (void)hipFree(x.second);
+  for (auto& x : free_) (void)hipFree(x.second);
+  for (auto& x : live_) (void)hipFree(x.first);
+}
+
+void* DevicePool::alloc(size_t bytes) {
+  if (bytes == 0) return nullptr;
+  SU2_CHECK(!detached_, kErrArgument, "device memory requested after its context was destroyed");
+  const size_t sz = pool_round(bytes);
+  auto take = [&]() -> void* {
+    auto it = free_.lower_bound(sz);
+    if (it == free_.end() || it->first > 2 * sz) return nullptr;
+    void* p = it->second;
+    live_[p] = it->first;
+    free_.erase(it);
+    return p;
+  };
+  if (void* p = take()) return p;
+  bool fits = false;
+  for (auto& x : held_) fits = fits || (x.first >= sz && x.first <= 2 * sz);
+  if (fits) {
+    recycle();
+    if (void* p = take()) return p;
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, sz) != hipSuccess) {
+    (void)hipGetLastError();
+    recycle();
+    release_free();
+    SU2_HIP(hipMalloc(&p, sz));
+  }
+  live_[p] = sz;
+  return p;
+}
+
+void DevicePool::free(void* p) {
+  if (!p) return;
+  auto it = live_.find(p);
+  if (detached_ || it == live_.end()) {   // context gone (or not a pool block)
+    if (it != live_.end()) live_.erase(it);
+    (void)hipSetDevice(device_);
+    (void)hipFree(p);
+    return;
+  }
+  held_.push_back({it->second, p});
+  live_.erase(it);
+}
+
+void DevicePool::set_stream(hipStream_t s) {
+  recycle();   // blocks held back for the old stream are released against it
+  stream_ = s;
+}
+
+void DevicePool::detach() {
+  recycle();
+  release_free();
+  stream_ = nullptr;
+  detached_ = true;
+}
+
+void DevicePool::recycle() {
+  if (held_.empty()) return;
+  SU2_HIP(hipStreamSynchronize(stream_));
+  for (auto& x : held_) free_.insert({x.first, x.second});
+  held_.clear();
+}
+
+void DevicePool::release_free() {
+  for (auto& x : free_) (void)hipFree(x.second);
+  free_.clear();
+}
 
 double* Context::reduce_scratch(size_t n) {
   if (n > d_red_n_) {
@@ -92,14 +183,15 @@ Tensor::Tensor(Context* ctx, std::vector<Leg> legs, int locspin, const std::vect
   }
   size_ = off;
   ctx_->make_current();
+  pool_ = ctx_->pool();
   if (size_ > 0) {
-    SU2_HIP(hipMalloc(&d_, size_ * sizeof(double)));
+    d_ = ctx_->dev_alloc_n<double>(size_);
     SU2_HIP(hipMemsetAsync(d_, 0, size_ * sizeof(double), ctx_->stream()));
   }
 }
 
 Tensor::~Tensor() {
-  if (d_) (void)hipFree(d_);
+  if (d_) pool_->free(d_);
 }
 
 int Tensor::find(const int* twoj) const {

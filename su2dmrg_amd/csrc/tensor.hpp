@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <memory>
 #include <vector>
 
 #include "structure.hpp"
@@ -31,6 +32,7 @@ struct Block {
 };
 
 class Context;
+class DevicePool;
 
 class Tensor {
  public:
@@ -63,6 +65,7 @@ class Tensor {
   std::vector<Block> blocks_;
   std::map<long long, int> by_key_;
   double* d_ = nullptr;
+  std::shared_ptr<DevicePool> pool_;   // outlives the context if need be
   long long size_ = 0;
 };
 
