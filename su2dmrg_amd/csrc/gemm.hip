@@ -25,13 +25,16 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
-// Logical work-unit id of this workgroup: the hardware deals workgroups to
-// the 8 XCDs round-robin (id % 8); give each XCD a contiguous run of logical
-// units so neighbouring tiles (which share A or B panels) meet in one L2.
-__device__ __forceinline__ long long xcd_remap(long long bid, long long nb) {
-  long long q = nb / 8, r = nb % 8;
-  long long x = bid % 8, local = bid / 8;
-  return x * q + (x < r ? x : r) + local;
+// Logical work unit of this workgroup, or -1: the hardware deals workgroups
+// to the 8 XCDs round-robin (id % 8).  XCD x takes the contiguous units
+// [begin[x], begin[x+1]), so that neighbouring tiles (which share A or B
+// panels) meet in one L2; the ranges hold equal work, not equal unit counts
+// (units of different problems differ in their K), and the grid is 8 times
+// the longest range, the surplus workgroups leaving at once.
+__device__ __forceinline__ long long xcd_unit(const XcdSplit& xs) {
+  const int x = (int)(blockIdx.x % 8);
+  const long long u = xs.begin[x] + blockIdx.x / 8;
+  return u < xs.begin[x + 1] ? u : -1;
 }
 
 __device__ __forceinline__ int find_problem(const GemmProblem* __restrict__ probs, int nprob,
@@ -275,13 +278,13 @@ template <int BM, int BN, int BK, int WM, int WN, int PAD>
 __global__ __launch_bounds__(WM * WN * 64, (WavesPerSimd<BM, BN, WM, WN>::value)) void
 grouped_gemm_f64_kernel(
     const GemmProblem* __restrict__ probs, int nprob, const GemmSlice* __restrict__ slices,
-    double* __restrict__ ws, long long total_units) {
+    double* __restrict__ ws, XcdSplit xs) {
   using T = Tile<BM, BN, BK, WM, WN, PAD>;
   __shared__ double As[2][BK * T::LDA_S];
   __shared__ double Bs[2][BK * T::LDB_S];
 
-  const long long unit = xcd_remap(blockIdx.x, gridDim.x);
-  if (unit >= total_units) return;
+  const long long unit = xcd_unit(xs);
+  if (unit < 0) return;
   const int p = find_problem(probs, nprob, unit);
   const GemmProblem P = probs[p];
   long long u = unit - P.unit_begin;
@@ -468,13 +471,13 @@ template <int BM, int BN, int BK, int WM, int WN, int PAD, int TA>
 __global__ __launch_bounds__(WM * WN * 64, (WavesPerSimd<BM, BN, WM, WN>::value)) void
 grouped_gemm_f64_uniform(const GemmProblem* __restrict__ probs, int nprob,
                          const GemmSlice* __restrict__ slices, double* __restrict__ ws,
-                         long long total_units) {
+                         XcdSplit xs) {
   using T = Tile<BM, BN, BK, WM, WN, PAD>;
   __shared__ double As0[BK * T::LDA_S], As1[BK * T::LDA_S];
   __shared__ double Bs0[BK * T::LDB_S], Bs1[BK * T::LDB_S];
 
-  const long long unit = xcd_remap(blockIdx.x, gridDim.x);
-  if (unit >= total_units) return;
+  const long long unit = xcd_unit(xs);
+  if (unit < 0) return;
   const int p = find_problem(probs, nprob, unit);
   const GemmProblem P = probs[p];
   long long u = unit - P.unit_begin;
@@ -624,7 +627,7 @@ __global__ void splitk_reduce_kernel(const ReduceProblem* __restrict__ reds, int
   R.C[row + (long long)col * R.ldc] = s;
 }
 
-typedef void (*GemmKernel)(const GemmProblem*, int, const GemmSlice*, double*, long long);
+typedef void (*GemmKernel)(const GemmProblem*, int, const GemmSlice*, double*, XcdSplit);
 
 struct VariantInfo {
   GemmKernel fn;   // any mix of slice layouts
@@ -785,6 +788,31 @@ void GroupedGemm::set_variant(int v) {
   }
   total_units_ = units;
   SU2_CHECK(total_units_ < (1ll << 31), kErrCapacity, "gemm: too many work units");
+  // XCD ranges of equal work: a unit of problem p costs its k-tiles
+  // ceil(kt_total / ksplit) plus about two for its prologue and epilogue
+  {
+    std::vector<double> w(run_probs_.size()), cum(run_probs_.size() + 1, 0.0);
+    for (size_t i = 0; i < run_probs_.size(); ++i) {
+      const GemmProblem& p = run_probs_[i];
+      w[i] = std::ceil((double)p.kt_total / p.ksplit) + 2.0;
+      cum[i + 1] = cum[i] + w[i] * p.tiles_m * p.tiles_n * p.ksplit;
+    }
+    split_.begin[0] = 0;
+    size_t i = 0;
+    for (int x = 1; x < 8; ++x) {
+      const double target = cum.back() * x / 8.0;
+      while (i + 1 < run_probs_.size() && cum[i + 1] <= target) ++i;
+      long long u = total_units_;
+      if (!run_probs_.empty())
+        u = run_probs_[i].unit_begin + (long long)std::ceil((target - cum[i]) / w[i]);
+      split_.begin[x] = std::max(split_.begin[x - 1], std::min(u, total_units_));
+    }
+    split_.begin[8] = total_units_;
+    long long longest = 0;
+    for (int x = 0; x < 8; ++x) longest = std::max(longest, split_.begin[x + 1] - split_.begin[x]);
+    grid_ = 8 * longest;
+    SU2_CHECK(grid_ < (1ll << 31), kErrCapacity, "gemm: grid of %lld workgroups", grid_);
+  }
   if (!run_probs_.empty()) {
     d_probs_ = (GemmProblem*)pool_->alloc(run_probs_.size() * sizeof(GemmProblem));
     SU2_HIP(hipMemcpy(d_probs_, run_probs_.data(), run_probs_.size() * sizeof(GemmProblem),
@@ -808,8 +836,8 @@ void GroupedGemm::launch(hipStream_t stream) const {
   if (total_units_ == 0) return;
   const VariantInfo& V = kVariants[variant_];
   const GemmKernel fn = layout_ == 0 ? V.fn_nn : layout_ == 1 ? V.fn_tn : V.fn;
-  hipLaunchKernelGGL(fn, dim3((unsigned)total_units_), dim3(V.threads), 0, stream, d_probs_,
-                     (int)run_probs_.size(), d_slices_, d_ws_, total_units_);
+  hipLaunchKernelGGL(fn, dim3((unsigned)grid_), dim3(V.threads), 0, stream, d_probs_,
+                     (int)run_probs_.size(), d_slices_, d_ws_, split_);
   SU2_HIP(hipGetLastError());
   if (!reds_.empty()) {
     long long blocks = (total_red_elems_ + 255) / 256;

@@ -43,6 +43,11 @@ struct GemmProblem {
   long long unit_begin;
 };
 
+// Unit ranges of the 8 XCDs: XCD x runs units [begin[x], begin[x+1]).
+struct XcdSplit {
+  long long begin[9];
+};
+
 struct ReduceProblem {
   double* C;
   long long ldc;
@@ -100,6 +105,8 @@ class GroupedGemm {
   ReduceProblem* d_reds_ = nullptr;
   double* d_ws_ = nullptr;
   long long total_units_ = 0;
+  XcdSplit split_{};
+  long long grid_ = 0;   // 8 x the longest XCD range
   long long total_red_elems_ = 0;
   double flops_ = 0;
   int target_units_ = 2048;
