@@ -762,12 +762,19 @@ void GroupedGemm::set_variant(int v) {
   }
   // split K so that the launch has about target_units comparable work units
   const double unit_work = std::max(1.0, work / target_units_);
+  for (auto& p : probs) {
+    int ks = (int)std::floor((double)p.kt_total / unit_work);
+    p.ksplit = std::max(1, std::min(ks, std::max(1, p.kt_total / 8)));
+  }
+  // units in order of decreasing cost (stable), so that the cheap units of a
+  // launch come last in the XCD ranges and fill their tails
+  std::stable_sort(probs.begin(), probs.end(), [](const GemmProblem& a, const GemmProblem& b) {
+    return (a.kt_total + a.ksplit - 1) / a.ksplit > (b.kt_total + b.ksplit - 1) / b.ksplit;
+  });
   long long units = 0, ws = 0;
   for (auto& p : probs) {
     if (p.M == 0 || p.N == 0) continue;
-    int ks = (int)std::floor((double)p.kt_total / unit_work);
-    ks = std::max(1, std::min(ks, std::max(1, p.kt_total / 8)));
-    p.ksplit = ks;
+    const int ks = p.ksplit;
     p.unit_begin = units;
     units += (long long)p.tiles_m * p.tiles_n * ks;
     if (ks > 1) {
