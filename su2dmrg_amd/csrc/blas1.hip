@@ -98,6 +98,59 @@ __global__ __launch_bounds__(kThreads) void strided_axpy_kernel(
   }
 }
 
+constexpr int kGatherChunk = 4096;   // bundle elements per workgroup
+
+__global__ __launch_bounds__(kThreads) void bundle_gather_kernel(
+    const BundleDesc* __restrict__ desc, const BundleSrc* __restrict__ table,
+    const AxpyChunk* __restrict__ chunks) {
+  const AxpyChunk ch = chunks[blockIdx.x];
+  const BundleDesc d = desc[ch.desc];
+  const int blk = d.rows * d.cols;
+  for (int j = threadIdx.x; j < ch.count; j += kThreads) {
+    const int e = ch.first + j;
+    int t, q;
+    if (d.interleaved) {
+      t = e % d.nt;
+      q = e / d.nt;
+    } else {
+      t = e / blk;
+      q = e - t * blk;
+    }
+    const BundleSrc* s = table + d.src0 + t * d.npos;
+    double acc = 0.0;
+    for (int p = 0; p < d.npos; ++p)
+      if (s[p].src) acc += s[p].alpha * s[p].src[q];
+    d.dst[e] = acc;
+  }
+}
+
+struct TileRef {
+  int desc, r0, c0, pad;
+};
+
+constexpr int kTile = 64;
+
+__global__ __launch_bounds__(kThreads) void transpose_kernel(const TransposeDesc* __restrict__ desc,
+                                                             const TileRef* __restrict__ tiles) {
+  __shared__ double t[kTile][kTile + 1];
+  const TileRef tr = tiles[blockIdx.x];
+  const TransposeDesc d = desc[tr.desc];
+  const int n = d.n;
+  const int lx = threadIdx.x % kTile, ly = threadIdx.x / kTile;
+  constexpr int kRowsPerPass = kThreads / kTile;
+  // t[c - c0][r - r0] = src[r + c*n], reads along r
+  for (int k = ly; k < kTile; k += kRowsPerPass) {
+    const int r = tr.r0 + lx, c = tr.c0 + k;
+    if (r < n && c < n) t[k][lx] = d.src[r + (long long)c * n];
+  }
+  __syncthreads();
+  // dst[r*n + c], writes along c
+  for (int k = ly; k < kTile; k += kRowsPerPass) {
+    const int r = tr.r0 + k, c = tr.c0 + lx;
+    if (r < n && c < n) d.dst[(long long)r * n + c] = t[lx][k];
+  }
+}
+
 unsigned grid_for(long long n) {
   long long b = (n + kThreads - 1) / kThreads;
   if (b > 4096) b = 4096;
@@ -191,6 +244,75 @@ void dev_strided_axpy_launches(Context& c,
 
 void dev_strided_axpy(Context& c, const std::vector<StridedAxpy>& desc) {
   dev_strided_axpy_launches(c, {desc});
+}
+
+}  // namespace su2
+
+namespace su2 {
+
+namespace {
+
+// Upload a descriptor table and a work table (each padded to 16 bytes) in one
+// block of the context's cache; the block goes back to the cache once the
+// stream has run past the launches that read it.
+template <class A, class B, class C>
+char* upload3(Context& c, const std::vector<A>& a, const std::vector<B>& b, const std::vector<C>& w,
+              size_t& ob, size_t& ow) {
+  auto pad = [](size_t x) { return (x + 15) / 16 * 16; };
+  ob = pad(a.size() * sizeof(A));
+  ow = ob + pad(b.size() * sizeof(B));
+  const size_t bytes = ow + w.size() * sizeof(C);
+  std::vector<char> host(bytes, 0);
+  if (!a.empty()) std::memcpy(host.data(), a.data(), a.size() * sizeof(A));
+  if (!b.empty()) std::memcpy(host.data() + ob, b.data(), b.size() * sizeof(B));
+  std::memcpy(host.data() + ow, w.data(), w.size() * sizeof(C));
+  char* dev = (char*)c.dev_alloc(bytes);
+  SU2_HIP(hipMemcpyAsync(dev, host.data(), bytes, hipMemcpyHostToDevice, c.stream()));
+  return dev;
+}
+
+}  // namespace
+
+void dev_bundle_gather(Context& c, const std::vector<BundleDesc>& desc,
+                       const std::vector<BundleSrc>& src_table) {
+  std::vector<AxpyChunk> chunks;
+  for (size_t i = 0; i < desc.size(); ++i) {
+    const BundleDesc& d = desc[i];
+    const long long n = (long long)d.nt * d.rows * d.cols;
+    if (n <= 0) continue;
+    SU2_CHECK(n < (1LL << 31), kErrCapacity, "bundle gather: bundle of %lld elements", n);
+    SU2_CHECK(d.npos >= 0 && d.src0 >= 0 &&
+                  (long long)d.src0 + (long long)d.nt * d.npos <= (long long)src_table.size(),
+              kErrArgument, "bundle gather: source table range out of bounds");
+    for (long long f = 0; f < n; f += kGatherChunk)
+      chunks.push_back(AxpyChunk{(int)i, (int)f, (int)std::min<long long>(kGatherChunk, n - f), 0});
+  }
+  if (chunks.empty()) return;
+  size_t ob = 0, ow = 0;
+  char* dev = upload3(c, desc, src_table, chunks, ob, ow);
+  hipLaunchKernelGGL(bundle_gather_kernel, dim3((unsigned)chunks.size()), dim3(kThreads), 0,
+                     c.stream(), (const BundleDesc*)dev, (const BundleSrc*)(dev + ob),
+                     (const AxpyChunk*)(dev + ow));
+  SU2_HIP(hipGetLastError());
+  c.dev_free(dev);
+}
+
+void dev_transpose_batch(Context& c, const std::vector<TransposeDesc>& desc) {
+  std::vector<TileRef> tiles;
+  for (size_t i = 0; i < desc.size(); ++i) {
+    const int n = desc[i].n;
+    for (int c0 = 0; c0 < n; c0 += kTile)
+      for (int r0 = 0; r0 < n; r0 += kTile) tiles.push_back(TileRef{(int)i, r0, c0, 0});
+  }
+  if (tiles.empty()) return;
+  SU2_CHECK(tiles.size() < (1u << 31), kErrCapacity, "transpose: %zu tiles", tiles.size());
+  size_t ob = 0, ow = 0;
+  const std::vector<char> none;
+  char* dev = upload3(c, desc, none, tiles, ob, ow);
+  hipLaunchKernelGGL(transpose_kernel, dim3((unsigned)tiles.size()), dim3(kThreads), 0, c.stream(),
+                     (const TransposeDesc*)dev, (const TileRef*)(dev + ow));
+  SU2_HIP(hipGetLastError());
+  c.dev_free(dev);
 }
 
 }  // namespace su2

@@ -41,4 +41,33 @@ struct StridedAxpy {
 void dev_strided_axpy_launches(Context& c, const std::vector<std::vector<StridedAxpy>>& launches);
 void dev_strided_axpy(Context& c, const std::vector<StridedAxpy>& desc);
 
+// Operator bundles: each descriptor holds nt summands of rows x cols
+// column-major blocks, element q = r + rows*c of summand t written once as
+//   blocked:     dst[q + rows*cols*t]
+//   interleaved: dst[t + nt*q]
+// with the value sum_{p < npos} alpha[t][p] * src[t][p][q], accumulated in p
+// order from zero (a null src is an absent summand).  The sources of a
+// descriptor are src_table[src0 + t*npos + p].  Threads walk dst in order, so
+// the writes coalesce and every element is written exactly once.  The call
+// does not wait.
+struct BundleDesc {
+  double* dst;
+  int src0, nt, npos, rows, cols, interleaved;   // nt * rows * cols < 2^31
+};
+struct BundleSrc {
+  const double* src;
+  double alpha;
+};
+void dev_bundle_gather(Context& c, const std::vector<BundleDesc>& desc,
+                       const std::vector<BundleSrc>& src_table);
+
+// Batched transpose of square n x n blocks: dst[r*n + c] = src[r + c*n],
+// through 64 x 64 tiles in LDS (both sides coalesce).  The call does not wait.
+struct TransposeDesc {
+  double* dst;
+  const double* src;
+  int n;
+};
+void dev_transpose_batch(Context& c, const std::vector<TransposeDesc>& desc);
+
 }  // namespace su2

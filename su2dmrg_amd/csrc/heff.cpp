@@ -67,11 +67,7 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
   // terms by operator leg: a transition (j -> j', k) bundles every term of
   // leg k, at its position in that list
   std::map<int, std::vector<int>> by_k;
-  std::vector<int> local(T, -1);
-  for (int ti = 0; ti < T; ++ti) {
-    local[ti] = (int)by_k[terms[ti].k].size();
-    by_k[terms[ti].k].push_back(ti);
-  }
+  for (int ti = 0; ti < T; ++ti) by_k[terms[ti].k].push_back(ti);
 
   // transitions (j -> j', k) present on both sides of at least one term
   struct Tr {
@@ -160,74 +156,47 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
   if (work > 0) d_work_ = c.dev_alloc_n<double>(work);
   dev_fill(c, d_work_, work, 0.0);
 
-  // bundle assembly: one batched strided-axpy launch per summand position, so
-  // coupling sums accumulate in the order given (deterministic); then H_L of
+  // bundle assembly: each bundle element is the sum of its coupling's
+  // summands in the order given (deterministic), written once; then H_L of
   // each sector transposed (source row r lands in column r)
-  std::vector<std::vector<StridedAxpy>> launches;
-  size_t maxside = 0;
-  for (auto& t : terms) maxside = std::max({maxside, t.left.size(), t.right.size()});
-  for (size_t pos = 0; pos < maxside; ++pos) {
-    std::vector<StridedAxpy> desc;
-    for (int ti = 0; ti < T; ++ti) {
-      const HeffTerm& t = terms[ti];
-      const int tl = local[ti];
-      for (auto& tr : trs) {
-        if (tr.k != t.k) continue;
-        if (pos < t.left.size()) {
-          const Tensor& op = *t.left[pos].second;
-          int id = op.find3(tr.ja, t.k, tr.jap);
-          if (id >= 0) {
-            StridedAxpy d{};
-            d.dst = d_work_ + tr.lb_off + (long long)tl * tr.Da * tr.Dap;
-            d.src = op.data() + op.blocks()[id].offset;
-            d.rs = 1;
-            d.cs = tr.Da;
-            d.ld = tr.Da;
-            d.rows = tr.Da;
-            d.cols = tr.Dap;
-            d.alpha = t.left[pos].first;
-            desc.push_back(d);
+  std::vector<BundleDesc> bundles;
+  std::vector<BundleSrc> table;
+  for (auto& tr : trs) {
+    const std::vector<int>& tk = by_k.at(tr.k);
+    for (int side = 0; side < 2; ++side) {
+      size_t npos = 0;
+      for (int ti : tk) npos = std::max(npos, (side ? terms[ti].right : terms[ti].left).size());
+      BundleDesc d{};
+      d.dst = d_work_ + (side ? tr.rb_off : tr.lb_off);
+      d.src0 = (int)table.size();
+      d.nt = tr.nt;
+      d.npos = (int)npos;
+      d.rows = side ? tr.Db : tr.Da;
+      d.cols = side ? tr.Dbp : tr.Dap;
+      d.interleaved = side;   // Rb rows run (term, row) with the term fastest
+      for (int ti : tk) {
+        const auto& ops = side ? terms[ti].right : terms[ti].left;
+        for (size_t pos = 0; pos < npos; ++pos) {
+          BundleSrc e{nullptr, 0.0};
+          if (pos < ops.size()) {
+            const Tensor& op = *ops[pos].second;
+            const int id = op.find3(tr.ja, tr.k, tr.jap);
+            if (id >= 0) e = BundleSrc{op.data() + op.blocks()[id].offset, ops[pos].first};
           }
-        }
-        if (pos < t.right.size()) {
-          const Tensor& op = *t.right[pos].second;
-          int id = op.find3(tr.ja, t.k, tr.jap);
-          if (id >= 0) {
-            StridedAxpy d{};
-            d.dst = d_work_ + tr.rb_off + tl;
-            d.src = op.data() + op.blocks()[id].offset;
-            d.rs = tr.nt;
-            d.cs = (long long)tr.nt * tr.Db;
-            d.ld = tr.Db;
-            d.rows = tr.Db;
-            d.cols = tr.Dbp;
-            d.alpha = t.right[pos].first;
-            desc.push_back(d);
-          }
+          table.push_back(e);
         }
       }
+      bundles.push_back(d);
     }
-    if (!desc.empty()) launches.push_back(std::move(desc));
   }
-  if (!hlt_off.empty()) {
-    std::vector<StridedAxpy> desc;
-    for (auto& kv : hlt_off) {
-      const Block& hb = hl->blocks()[hl->find2(kv.first, kv.first)];
-      const int D = hb.dim[0];
-      StridedAxpy d{};
-      d.dst = d_work_ + kv.second;
-      d.src = hl->data() + hb.offset;
-      d.rs = D;
-      d.cs = 1;
-      d.ld = D;
-      d.rows = D;
-      d.cols = D;
-      d.alpha = 1.0;
-      desc.push_back(d);
-    }
-    launches.push_back(std::move(desc));
+  SU2_CHECK(table.size() < (1u << 31), kErrCapacity, "H_eff: %zu bundle summands", table.size());
+  dev_bundle_gather(c, bundles, table);
+  std::vector<TransposeDesc> transposes;
+  for (auto& kv : hlt_off) {
+    const Block& hb = hl->blocks()[hl->find2(kv.first, kv.first)];
+    transposes.push_back(TransposeDesc{d_work_ + kv.second, hl->data() + hb.offset, hb.dim[0]});
   }
-  dev_strided_axpy_launches(c, launches);
+  dev_transpose_batch(c, transposes);
 
   // stage 1: X(j->j') = f * Lb^T psi(j)   [T*D_a(j') x D_b(j)]
   for (auto& tr : trs) {
