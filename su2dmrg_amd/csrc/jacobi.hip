@@ -62,20 +62,34 @@ __device__ __forceinline__ void slice_rows(int L, int P, int s, int& lo, int& hi
   hi = min(L, lo + per);
 }
 
+// Upper-triangle tile u (row-major over tr <= tc) of a TT x TT tile grid.
+__device__ __forceinline__ void upper_tile(int u, int TT, int& tr, int& tc) {
+  tr = 0;
+  while (u >= TT - tr) {
+    u -= TT - tr;
+    ++tr;
+  }
+  tc = tr + u;
+}
+
 // Partial Gram matrix of one pair over one row slice of W, on the matrix
-// cores.  32-row chunks of the pair's NP = 2 CB columns are staged in LDS
-// (columns >= n zero); the (NP/16)^2 output tiles of 16 x 16 are spread over
-// the waves, each tile a v_mfma_f64_16x16x4f64 chain: A = X^T fragment
-// (lane l: X[r + l/16][16 tr + l%16]), B = X fragment (lane l:
-// X[r + l/16][16 tc + l%16]), D lane l: G[16 tr + l/16 + 4 q][16 tc + l%16].
-// The NP x NP result goes row-major to gpart[blockIdx.x] (workgroup = job *
-// pmax + slice).
-template <int CB, int NT>
+// cores.  RCH-row chunks of the pair's NP = 2 CB columns are staged in LDS
+// (columns >= n zero).  The Gram matrix is symmetric, so only the
+// TT (TT + 1) / 2 tiles of 16 x 16 on and above the diagonal (TT = NP / 16)
+// are computed, spread over the waves, each a v_mfma_f64_16x16x4f64 chain:
+// A = X^T fragment (lane l: X[r + l/16][16 tr + l%16]), B = X fragment (lane
+// l: X[r + l/16][16 tc + l%16]), D lane l: G[16 tr + l/16 + 4 q][16 tc +
+// l%16].  Each product X[r][a] X[r][b] is the same number either way round,
+// so the mirrored tile is the one its own chain would give.  The NP x NP
+// result goes row-major to gpart[blockIdx.x] (workgroup = job * pmax +
+// slice).
+template <int CB, int NT, int RCH>
 __global__ __launch_bounds__(NT) void gram_partial_kernel(const BlockDesc* __restrict__ blocks,
                                                           const PairJob* __restrict__ jobs,
                                                           int pmax, double* __restrict__ gpart) {
-  constexpr int NP = 2 * CB, TT = NP / 16, NW = NT / 64, TPW = TT * TT / NW;
-  static_assert(TPW * NW == TT * TT, "tiles must split evenly over the waves");
+  constexpr int NP = 2 * CB, TT = NP / 16, NU = TT * (TT + 1) / 2, NW = NT / 64;
+  constexpr int TPW = (NU + NW - 1) / NW;   // tiles per wave (the last waves may hold none)
+  static_assert(NW * 64 == NT, "whole waves");
   __shared__ double X[RCH][NP + 1];
   __shared__ int colmap[NP];
   const int job = blockIdx.x / pmax, slice = blockIdx.x - job * pmax;
@@ -87,11 +101,15 @@ __global__ __launch_bounds__(NT) void gram_partial_kernel(const BlockDesc* __res
   for (int c = t; c < n; c += NT) colmap[c] = c < J.bp ? J.p0 + c : J.q0 + (c - J.bp);
   int lo, hi;
   slice_rows(B.L, J.P, slice, lo, hi);
+  int ttr[TPW], ttc[TPW];
+#pragma unroll
+  for (int j = 0; j < TPW; ++j) upper_tile(min(wave * TPW + j, NU - 1), TT, ttr[j], ttc[j]);
   f64x4 acc[TPW];
 #pragma unroll
   for (int j = 0; j < TPW; ++j) acc[j] = f64x4{0.0, 0.0, 0.0, 0.0};
   // the next chunk's global loads are issued before the current chunk's MFMAs
   constexpr int LPT = RCH * NP / NT;
+  static_assert(LPT * NT == RCH * NP, "chunk elements per thread");
   double pre[LPT];
   auto fetch = [&](int r0) {
     const int nr = min(RCH, hi - r0);
@@ -115,21 +133,25 @@ __global__ __launch_bounds__(NT) void gram_partial_kernel(const BlockDesc* __res
     for (int k4 = 0; k4 < RCH; k4 += 4) {
       const int row = k4 + (lane >> 4);
 #pragma unroll
-      for (int j = 0; j < TPW; ++j) {
-        const int tau = wave * TPW + j, tr = tau / TT, tc = tau % TT;
-        acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(X[row][16 * tr + (lane & 15)],
-                                                      X[row][16 * tc + (lane & 15)], acc[j], 0, 0, 0);
-      }
+      for (int j = 0; j < TPW; ++j)
+        if (wave * TPW + j < NU)
+          acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(X[row][16 * ttr[j] + (lane & 15)],
+                                                        X[row][16 * ttc[j] + (lane & 15)], acc[j],
+                                                        0, 0, 0);
     }
     __syncthreads();
   }
   double* out = gpart + (size_t)blockIdx.x * NP * NP;
 #pragma unroll
   for (int j = 0; j < TPW; ++j) {
-    const int tau = wave * TPW + j, tr = tau / TT, tc = tau % TT;
+    if (wave * TPW + j >= NU) continue;
+    const int tr = ttr[j], tc = ttc[j];
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-      out[(16 * tr + (lane >> 4) + 4 * q) * NP + 16 * tc + (lane & 15)] = acc[j][q];
+    for (int q = 0; q < 4; ++q) {
+      const int a = 16 * tr + (lane >> 4) + 4 * q, b = 16 * tc + (lane & 15);
+      out[a * NP + b] = acc[j][q];
+      if (tr != tc) out[b * NP + a] = acc[j][q];
+    }
   }
 }
 
@@ -727,13 +749,13 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
       const int pm = round_pmax[r];
       const PairJob* jr = (const PairJob*)djobs.p + round_off[r];
       if (CB == 16) {
-        hipLaunchKernelGGL((gram_partial_kernel<16, 64>), dim3(nj * pm), dim3(64), 0, st,
+        hipLaunchKernelGGL((gram_partial_kernel<16, 64, RCH>), dim3(nj * pm), dim3(64), 0, st,
                            (const BlockDesc*)ddesc.p, jr, pm, (double*)dgpart.p);
         hipLaunchKernelGGL((jacobi_rotate_kernel<16, 64>), dim3(nj * pm), dim3(64), 0, st,
                            (const BlockDesc*)ddesc.p, jr, pm, (const double*)dgpart.p, tol, 1e-14,
                            (unsigned long long*)doff.p, inner_sweeps);
       } else {
-        hipLaunchKernelGGL((gram_partial_kernel<32, 256>), dim3(nj * pm), dim3(256), 0, st,
+        hipLaunchKernelGGL((gram_partial_kernel<32, 1024, 64>), dim3(nj * pm), dim3(1024), 0, st,
                            (const BlockDesc*)ddesc.p, jr, pm, (double*)dgpart.p);
         hipLaunchKernelGGL((jacobi_rotate_kernel<32, 512>), dim3(nj * pm), dim3(512), 0, st,
                            (const BlockDesc*)ddesc.p, jr, pm, (const double*)dgpart.p, tol, 1e-14,
