@@ -63,7 +63,8 @@ __global__ __launch_bounds__(kThreads) void dot_partial_kernel(const double* __r
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-__global__ __launch_bounds__(kThreads) void dot_final_kernel(double* __restrict__ part) {
+__global__ __launch_bounds__(kThreads) void dot_final_kernel(const double* __restrict__ part,
+                                                             double* __restrict__ res) {
   __shared__ double sh[kThreads];
   double s = 0.0;
   for (int i = threadIdx.x; i < kDotBlocks; i += kThreads) s += part[i];
@@ -73,7 +74,16 @@ __global__ __launch_bounds__(kThreads) void dot_final_kernel(double* __restrict_
     if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[kDotBlocks] = sh[0];
+  if (threadIdx.x == 0) *res = sh[0];
+}
+
+// y += (sign * a[0]) x with the coefficient read on the device
+__global__ void axpy_dev_kernel(double* __restrict__ y, const double* __restrict__ x, long long n,
+                                const double* __restrict__ a, double sign) {
+  const double av = sign * a[0];
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) y[i] += av * x[i];
 }
 
 constexpr int kAxpyChunk = 4096;   // elements of one descriptor per workgroup
@@ -197,12 +207,34 @@ double dev_dot(Context& c, const double* x, const double* y, long long n) {
   hipLaunchKernelGGL(dot_partial_kernel, dim3(kDotBlocks), dim3(kThreads), 0, c.stream(), x, y, n,
                      part);
   SU2_HIP(hipGetLastError());
-  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, c.stream(), part);
+  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, c.stream(), part,
+                     part + kDotBlocks);
   SU2_HIP(hipGetLastError());
   double* h = c.host_scratch(1);
   SU2_HIP(hipMemcpyAsync(h, part + kDotBlocks, sizeof(double), hipMemcpyDeviceToHost, c.stream()));
   SU2_HIP(hipStreamSynchronize(c.stream()));
   return h[0];
+}
+
+void dev_dot_to(Context& c, const double* x, const double* y, long long n, double* res) {
+  if (n <= 0) {
+    dev_fill(c, res, 1, 0.0);
+    return;
+  }
+  double* part = c.reduce_scratch(kDotBlocks + 1);
+  hipLaunchKernelGGL(dot_partial_kernel, dim3(kDotBlocks), dim3(kThreads), 0, c.stream(), x, y, n,
+                     part);
+  SU2_HIP(hipGetLastError());
+  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, c.stream(), part, res);
+  SU2_HIP(hipGetLastError());
+}
+
+void dev_axpy_dev(Context& c, double* y, const double* x, long long n, const double* a,
+                  double sign) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(axpy_dev_kernel, dim3(grid_for(n)), dim3(kThreads), 0, c.stream(), y, x, n, a,
+                     sign);
+  SU2_HIP(hipGetLastError());
 }
 
 void dev_strided_axpy_launches(Context& c,

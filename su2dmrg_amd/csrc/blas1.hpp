@@ -21,6 +21,11 @@ void dev_axpy(Context& c, double* y, const double* x, long long n, double a);   
 void dev_axpby(Context& c, double* y, const double* x, long long n, double a, double b);  // y = a x + b y
 // sum_i x_i y_i (synchronises the context stream and returns on the host)
 double dev_dot(Context& c, const double* x, const double* y, long long n);
+// the same sum, left on the device in *res (stream-ordered, no wait)
+void dev_dot_to(Context& c, const double* x, const double* y, long long n, double* res);
+// y += (sign * a[0]) x, the coefficient read on the device (no wait)
+void dev_axpy_dev(Context& c, double* y, const double* x, long long n, const double* a,
+                  double sign);
 
 // Batched strided scale-add: for each descriptor d,
 //   dst_d[r*rs + c*cs] += alpha_d * src_d[r + c*ld],  r < rows_d, c < cols_d

@@ -371,27 +371,43 @@ Dmrg::StepOut Dmrg::step(int il, int ir, std::unique_ptr<Tensor> psi0, double p1
   // lower states in this two-site basis (adjacent sites only, dmrg_su2.cpp:611)
   std::vector<std::unique_ptr<Tensor>> phis;
   if (!lower_.empty() && il + 1 == ir) phis = lower_vectors(il, ir, o.FL, o.FR, *psi);
-  hipEvent_t e0, e1;
-  SU2_HIP(hipEventCreate(&e0));
-  SU2_HIP(hipEventCreate(&e1));
+  // matvec device time from an event pair per matvec, read once the solve
+  // has finished (no wait per matvec)
+  std::vector<hipEvent_t> ev;
+  auto release_events = [&]() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    ev.clear();
+  };
   float mv_ms = 0;
   auto apply = [&](const double* x, double* y) {
+    hipEvent_t e0, e1;
+    SU2_HIP(hipEventCreate(&e0));
+    ev.push_back(e0);
+    SU2_HIP(hipEventCreate(&e1));
+    ev.push_back(e1);
     SU2_HIP(hipEventRecord(e0, c.stream()));
     plan.apply_raw(x, y);
     SU2_HIP(hipEventRecord(e1, c.stream()));
-    SU2_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    SU2_HIP(hipEventElapsedTime(&ms, e0, e1));
-    mv_ms += ms;
     // - sum_j E_j <phi_j|x> phi_j (dmrg_su2.cpp:611-623)
     for (size_t j = 0; j < phis.size(); ++j) {
       const double ov = dev_dot(c, phis[j]->data(), x, psi->size());
       dev_axpy(c, y, phis[j]->data(), psi->size(), -lower_[j]->energy * ov);
     }
   };
-  LanczosResult lr = lanczos_ground(c, apply, psi->data(), psi->size(), mlanc);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  LanczosResult lr;
+  try {
+    lr = lanczos_ground(c, apply, psi->data(), psi->size(), mlanc);
+    SU2_HIP(hipStreamSynchronize(c.stream()));
+    for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+      float ms = 0;
+      SU2_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+      mv_ms += ms;
+    }
+  } catch (...) {
+    release_events();
+    throw;
+  }
+  release_events();
   stats_.t_matvec += mv_ms * 1e-3;
   stats_.matvecs += lr.matvecs;
   stats_.heff_flops += plan.algorithmic_flops() * lr.matvecs;

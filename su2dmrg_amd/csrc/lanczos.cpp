@@ -74,6 +74,11 @@ LanczosResult lanczos_ground(Context& c, const std::function<void(const double*,
   double* tmp = nullptr;
   for (auto& p : ff) p = c.dev_alloc_n<double>(std::max<long long>(n, 1));
   tmp = c.dev_alloc_n<double>(std::max<long long>(n, 1));
+  // alphas of the main loop stay on the device until a Ritz step needs them
+  // (indices [1, dev_hi)); slot mlanc holds the current reorthogonalisation
+  // coefficient.  Only the residual norm, which steers the loop, is waited for.
+  double* d_coef = c.dev_alloc_n<double>(mlanc + 1);
+  int dev_hi = 1;
   std::vector<double> aal(mlanc, 0.0), nnl(mlanc + 1, 0.0);
   auto hv = [&](const double* x, double* y) {
     apply(x, y);
@@ -82,6 +87,11 @@ LanczosResult lanczos_ground(Context& c, const std::function<void(const double*,
   auto nrm = [&](const double* x) { return std::sqrt(dev_dot(c, x, x, n)); };
   // Ritz vector of the m x m tridiagonal into psi, its value returned
   auto ritz = [&](int m) {
+    if (dev_hi > 1) {
+      SU2_HIP(hipMemcpyAsync(aal.data() + 1, d_coef + 1, sizeof(double) * (dev_hi - 1),
+                             hipMemcpyDeviceToHost, c.stream()));
+      SU2_HIP(hipStreamSynchronize(c.stream()));
+    }
     std::vector<double> d(aal.begin(), aal.begin() + m), e(nnl.begin() + 1, nnl.begin() + m), w, z;
     tridiag_eig(d, e, w, z);
     if (z[0] < 0)   // Ritz vector with a positive overlap on the start vector
@@ -101,6 +111,7 @@ LanczosResult lanczos_ground(Context& c, const std::function<void(const double*,
   auto finish = [&]() {
     for (auto p : ff) c.dev_free(p);
     c.dev_free(tmp);
+    c.dev_free(d_coef);
   };
 
   // ff[0] = normalised start vector (initialize_lanczos, lanczos_su2.cpp:36-47)
@@ -132,13 +143,14 @@ LanczosResult lanczos_ground(Context& c, const std::function<void(const double*,
   }
   for (int m = 2; m < mlanc; ++m) {
     hv(ff[m - 1], ff[m]);
-    aal[m - 1] = dev_dot(c, ff[m - 1], ff[m], n);
-    dev_axpy(c, ff[m], ff[m - 1], n, -aal[m - 1]);
+    dev_dot_to(c, ff[m - 1], ff[m], n, d_coef + (m - 1));
+    dev_axpy_dev(c, ff[m], ff[m - 1], n, d_coef + (m - 1), -1.0);
     dev_axpy(c, ff[m], ff[m - 2], n, -nnl[m - 1]);
     for (int j = 0; j < m; ++j) {
-      const double q = dev_dot(c, ff[m], ff[j], n);
-      dev_axpy(c, ff[m], ff[j], n, -q);
+      dev_dot_to(c, ff[m], ff[j], n, d_coef + mlanc);
+      dev_axpy_dev(c, ff[m], ff[j], n, d_coef + mlanc, -1.0);
     }
+    dev_hi = m;
     nnl[m] = nrm(ff[m]);
     if (nnl[m] > 0) dev_scal(c, ff[m], n, 1.0 / nnl[m]);
     if (nnl[m] < tol) {
