@@ -161,6 +161,34 @@ __global__ __launch_bounds__(kThreads) void transpose_kernel(const TransposeDesc
   }
 }
 
+// Strided scale-add of a descriptor that writes along its columns (cs == 1,
+// rs > 1): a 64 x 64 tile is read along r into LDS and added along c, so
+// both the reads and the writes coalesce.
+__global__ __launch_bounds__(kThreads) void strided_axpy_tiled_kernel(
+    const StridedAxpy* __restrict__ desc, const TileRef* __restrict__ tiles) {
+  __shared__ double t[kTile][kTile + 1];
+  const TileRef tr = tiles[blockIdx.x];
+  const StridedAxpy d = desc[tr.desc];
+  const int lx = threadIdx.x % kTile, ly = threadIdx.x / kTile;
+  constexpr int kRowsPerPass = kThreads / kTile;
+  for (int k = ly; k < kTile; k += kRowsPerPass) {
+    const int r = tr.r0 + lx, c = tr.c0 + k;
+    if (r < d.rows && c < d.cols) t[k][lx] = d.src[r + (long long)c * d.ld];
+  }
+  __syncthreads();
+  for (int k = ly; k < kTile; k += kRowsPerPass) {
+    const int r = tr.r0 + k, c = tr.c0 + lx;
+    if (r < d.rows && c < d.cols) {
+      // rounded product, then the sum (not fused): the arithmetic of
+      // strided_axpy_kernel
+#pragma clang fp contract(off)
+      double* p = d.dst + (r * d.rs + c * d.cs);
+      const double v = d.alpha * t[lx][k];
+      *p += v;
+    }
+  }
+}
+
 unsigned grid_for(long long n) {
   long long b = (n + kThreads - 1) / kThreads;
   if (b > 4096) b = 4096;
@@ -241,35 +269,59 @@ void dev_strided_axpy_launches(Context& c,
                                const std::vector<std::vector<StridedAxpy>>& launches) {
   std::vector<StridedAxpy> flat;
   std::vector<AxpyChunk> chunks;
-  std::vector<std::pair<size_t, size_t>> span;   // chunk range of each launch
+  std::vector<TileRef> tiles;
+  // chunk and tile ranges of each launch
+  std::vector<std::pair<size_t, size_t>> span, tspan;
   for (auto& l : launches) {
-    const size_t c0 = chunks.size();
+    const size_t c0 = chunks.size(), t0 = tiles.size();
     for (const StridedAxpy& d : l) {
       const long long n = (long long)d.rows * d.cols;
       if (n <= 0) continue;
       SU2_CHECK(n < (1LL << 31), kErrCapacity, "strided axpy: block of %lld elements", n);
       const int di = (int)flat.size();
       flat.push_back(d);
-      for (long long f = 0; f < n; f += kAxpyChunk)
-        chunks.push_back(AxpyChunk{di, (int)f, (int)std::min<long long>(kAxpyChunk, n - f), 0});
+      if (d.src && d.cs == 1 && d.rs > 1 && d.rows > 1 && d.cols > 1) {
+        for (int cc = 0; cc < d.cols; cc += kTile)
+          for (int r0 = 0; r0 < d.rows; r0 += kTile) tiles.push_back(TileRef{di, r0, cc, 0});
+      } else {
+        for (long long f = 0; f < n; f += kAxpyChunk)
+          chunks.push_back(AxpyChunk{di, (int)f, (int)std::min<long long>(kAxpyChunk, n - f), 0});
+      }
     }
     span.push_back({c0, chunks.size()});
+    tspan.push_back({t0, tiles.size()});
   }
-  if (chunks.empty()) return;
+  if (chunks.empty() && tiles.empty()) return;
+  SU2_CHECK(tiles.size() < (1u << 31), kErrCapacity, "strided axpy: %zu tiles", tiles.size());
   const size_t dbytes = (flat.size() * sizeof(StridedAxpy) + 15) / 16 * 16;
-  const size_t bytes = dbytes + chunks.size() * sizeof(AxpyChunk);
-  std::vector<char> host(bytes);
+  const size_t cbytes = (chunks.size() * sizeof(AxpyChunk) + 15) / 16 * 16;
+  const size_t bytes = dbytes + cbytes + tiles.size() * sizeof(TileRef);
+  std::vector<char> host(bytes, 0);
   std::memcpy(host.data(), flat.data(), flat.size() * sizeof(StridedAxpy));
-  std::memcpy(host.data() + dbytes, chunks.data(), chunks.size() * sizeof(AxpyChunk));
+  if (!chunks.empty())
+    std::memcpy(host.data() + dbytes, chunks.data(), chunks.size() * sizeof(AxpyChunk));
+  if (!tiles.empty())
+    std::memcpy(host.data() + dbytes + cbytes, tiles.data(), tiles.size() * sizeof(TileRef));
   char* dev = (char*)c.dev_alloc(bytes);
   SU2_HIP(hipMemcpyAsync(dev, host.data(), bytes, hipMemcpyHostToDevice, c.stream()));
   const StridedAxpy* d_desc = (const StridedAxpy*)dev;
   const AxpyChunk* d_chunks = (const AxpyChunk*)(dev + dbytes);
-  for (auto& s : span) {
-    if (s.second == s.first) continue;
-    hipLaunchKernelGGL(strided_axpy_kernel, dim3((unsigned)(s.second - s.first)), dim3(kThreads),
-                       0, c.stream(), d_desc, d_chunks + s.first);
-    SU2_HIP(hipGetLastError());
+  const TileRef* d_tiles = (const TileRef*)(dev + dbytes + cbytes);
+  // the descriptors of one launch write disjoint elements, so its chunked
+  // and tiled parts may run in either order
+  for (size_t i = 0; i < span.size(); ++i) {
+    const auto& s = span[i];
+    const auto& ts = tspan[i];
+    if (s.second > s.first) {
+      hipLaunchKernelGGL(strided_axpy_kernel, dim3((unsigned)(s.second - s.first)),
+                         dim3(kThreads), 0, c.stream(), d_desc, d_chunks + s.first);
+      SU2_HIP(hipGetLastError());
+    }
+    if (ts.second > ts.first) {
+      hipLaunchKernelGGL(strided_axpy_tiled_kernel, dim3((unsigned)(ts.second - ts.first)),
+                         dim3(kThreads), 0, c.stream(), d_desc, d_tiles + ts.first);
+      SU2_HIP(hipGetLastError());
+    }
   }
   c.dev_free(dev);
 }
