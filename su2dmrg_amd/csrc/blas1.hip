@@ -110,11 +110,41 @@ __global__ __launch_bounds__(kThreads) void strided_axpy_kernel(
 
 constexpr int kGatherChunk = 4096;   // bundle elements per workgroup
 
+constexpr int kGatherMaxPos = 16;    // summands held in registers (q-chunks)
+
+// A chunk is either elements [first, first + count) of the bundle (pad 0) or,
+// for an interleaved bundle of at most kThreads terms and kGatherMaxPos
+// summands, source positions q in [first, first + count) of every term
+// (pad 1): each thread then keeps one term, its sources and couplings in
+// registers, and walks q; lanes still write consecutive elements.
 __global__ __launch_bounds__(kThreads) void bundle_gather_kernel(
     const BundleDesc* __restrict__ desc, const BundleSrc* __restrict__ table,
     const AxpyChunk* __restrict__ chunks) {
   const AxpyChunk ch = chunks[blockIdx.x];
   const BundleDesc d = desc[ch.desc];
+  if (ch.pad) {
+    const int nt = d.nt, G = kThreads / nt;
+    const int i = threadIdx.x;
+    if (i >= G * nt) return;   // no barrier follows
+    const int t = i % nt, g = i / nt;
+    const BundleSrc* s = table + d.src0 + t * d.npos;
+    const double* sp[kGatherMaxPos];
+    double al[kGatherMaxPos];
+#pragma unroll
+    for (int p = 0; p < kGatherMaxPos; ++p) {
+      sp[p] = p < d.npos ? s[p].src : nullptr;
+      al[p] = p < d.npos ? s[p].alpha : 0.0;
+    }
+    for (int k = g; k < ch.count; k += G) {
+      const int q = ch.first + k;
+      double acc = 0.0;
+#pragma unroll
+      for (int p = 0; p < kGatherMaxPos; ++p)
+        if (sp[p]) acc += al[p] * sp[p][q];
+      d.dst[t + nt * q] = acc;
+    }
+    return;
+  }
   const int blk = d.rows * d.cols;
   for (int j = threadIdx.x; j < ch.count; j += kThreads) {
     const int e = ch.first + j;
@@ -368,8 +398,16 @@ void dev_bundle_gather(Context& c, const std::vector<BundleDesc>& desc,
     SU2_CHECK(d.npos >= 0 && d.src0 >= 0 &&
                   (long long)d.src0 + (long long)d.nt * d.npos <= (long long)src_table.size(),
               kErrArgument, "bundle gather: source table range out of bounds");
-    for (long long f = 0; f < n; f += kGatherChunk)
-      chunks.push_back(AxpyChunk{(int)i, (int)f, (int)std::min<long long>(kGatherChunk, n - f), 0});
+    if (d.interleaved && d.nt <= kThreads && d.npos <= kGatherMaxPos) {
+      const long long nq = (long long)d.rows * d.cols;
+      const long long qc = std::max(1, kGatherChunk / d.nt);
+      for (long long q0 = 0; q0 < nq; q0 += qc)
+        chunks.push_back(AxpyChunk{(int)i, (int)q0, (int)std::min<long long>(qc, nq - q0), 1});
+    } else {
+      for (long long f = 0; f < n; f += kGatherChunk)
+        chunks.push_back(
+            AxpyChunk{(int)i, (int)f, (int)std::min<long long>(kGatherChunk, n - f), 0});
+    }
   }
   if (chunks.empty()) return;
   size_t ob = 0, ow = 0;
