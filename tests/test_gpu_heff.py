@@ -100,6 +100,35 @@ def test_heff_presummed_sides(ctx):
     assert _relerr(got, ref) < RTOL
 
 
+def test_heff_many_summands(ctx):
+    # Both terms share the interleaved right bundle of each transition, whose
+    # summand count is the longer side (17): more than the bundle gather holds
+    # in registers per term, so the bundle is assembled element by element,
+    # with the shorter side's missing summands as absent entries.
+    from su2dmrg_amd import lib as L
+    _say("heff with 17 right summands")
+    work = synthetic.small(seed=23, twoj=(1, 3, 5), dims=(40, 60, 20), nterm=18)
+    twoj, da, db = work["twoj"], work["dims_a"], work["dims_b"]
+    Ls = [t[0] for t in work["terms"]]
+    Rs = [t[1] for t in work["terms"]]
+    psi = L.wavefunction(ctx, twoj, da, db, work["psi"])
+    out = L.wavefunction(ctx, twoj, da, db)
+    tl = [L.vector_operator(ctx, twoj, da, x) for x in Ls[:2]]
+    tr = [L.vector_operator(ctx, twoj, db, x) for x in Rs]
+    rng = np.random.default_rng(7)
+    c0 = list(rng.uniform(-1.5, 1.5, 17))
+    c1 = list(rng.uniform(-1.5, 1.5, 3))
+    plan = L.HeffPlan(ctx, psi, None, None,
+                      [([(1.0, tl[0])], [(c, t) for c, t in zip(c0, tr[:17])]),
+                       ([(1.0, tl[1])], [(c, t) for c, t in zip(c1, tr[15:18])])])
+    plan.apply(psi, out)
+    got = L.unpack_wavefunction(out, out.download())
+    terms = [(Ls[0], oheff.presum(list(zip(c0, Rs[:17])))),
+             (Ls[1], oheff.presum(list(zip(c1, Rs[15:18]))))]
+    ref = oheff.heff_apply(work["psi"], None, None, terms)
+    assert _relerr(got, ref) < RTOL
+
+
 def test_heff_deterministic(ctx):
     from su2dmrg_amd import lib as L
     _say("heff determinism")
