@@ -36,6 +36,19 @@ typedef struct su2_ctx su2_ctx;
 typedef struct su2_tensor su2_tensor;
 typedef struct su2_heff su2_heff;
 typedef struct su2_dmrg su2_dmrg;
+typedef struct su2_batch su2_batch;
+typedef struct su2_server su2_server;
+
+/* One served job (the flags of one reference run, main_omp.cpp:42-132:
+ * -lx -ly -jcoup -qcoup -boundary -sec -d -exci and the sweep count). */
+typedef struct su2_job {
+  int lx, ly;
+  double jcoup, qcoup;
+  int bdry, totspin, max_dcut;
+  int exci;      /* 0: ground state; k > 0: k-th excited state of the sector */
+  int nsweep;    /* sweeps after the warm-up, per level */
+  double stop;   /* |dE| early stop between sweeps (main_omp.cpp:197); < 0: off */
+} su2_job;
 
 enum su2_status {
   SU2_OK = 0,
@@ -55,6 +68,10 @@ int su2_ctx_create(int device, su2_ctx** out);
 int su2_ctx_destroy(su2_ctx* ctx);
 const char* su2_last_error(const su2_ctx* ctx);
 int su2_ctx_synchronize(su2_ctx* ctx);
+/* Device memory of the context's cache: bytes in use, the most in use at
+ * once since creation (or the last reset_peak = 1), bytes cached for reuse. */
+int su2_ctx_memory(su2_ctx* ctx, long long* live, long long* peak, long long* cached,
+                   int reset_peak);
 /* Tuning of the one-sided Jacobi SVD behind su2_svd_truncate and the DMRG
  * solves: column block width 16 or 32 (default 32), inner Jacobi sweeps per
  * visit of a pair 1..3 (default 1).  No counterpart in the reference; the
@@ -126,6 +143,15 @@ int su2_heff_autotune(su2_ctx* ctx, su2_heff* plan, int reps, int* chosen, float
                       int* nvariants);
 /* Pin the tile variants of the two launches (e.g. to a tuned choice). */
 int su2_heff_set_variants(su2_ctx* ctx, su2_heff* plan, int stage1, int stage2);
+/* Arithmetic of su2_heff_apply: bits = 64 (default) or 32.  32 runs both
+ * grouped launches on FP32 MFMA (v_mfma_f32_16x16x4_f32) with FP32
+ * accumulation, on FP32 copies of the plan's operator bundles and block
+ * Hamiltonians made on first use; psi is rounded to FP32 and the result
+ * widened to FP64 (relative error about 1e-6).  The early sweeps of a
+ * mixed-precision run use it (su2_dmrg_set_matvec_precision); the reference
+ * computes in FP64 only (tensor_dmrg_src.cpp:536-575).  Tile variants,
+ * autotuning and su2_heff_time then apply to the FP32 launches. */
+int su2_heff_set_precision(su2_ctx* ctx, su2_heff* plan, int bits);
 /* Kernel path of the two launches: uniform = 1 (default) runs a launch whose
  * slices share one operand layout (stage 1: op(A) = A^T, stage 2: op(A) = A)
  * with the straight-line kernel, 0 with the general kernel; both give the
@@ -170,8 +196,8 @@ int su2_recoupling(int kind, const int* labels, double* out);
 /* Two-site SU(2) DMRG on an lx x ly cylinder (open x, periodic y) for total
  * spin 2S = totspin, keeping max_dcut multiplets (dmrg_su2::dmrg_su2,
  * dmrg_su2.cpp:31-310; flags -lx -ly -jcoup -qcoup -boundary -sec -d of
- * main_omp.cpp:42-132).  Only qcoup = 0 is supported (no plaquette terms);
- * boundary codes 0, 1, 2.  All environments, MPS projectors and Lanczos
+ * main_omp.cpp:42-132), with the J-Q plaquette terms when qcoup != 0
+ * (dmrg_su2.cpp:200-232, 635-655); boundary codes 0, 1, 2.  All environments, MPS projectors and Lanczos
  * vectors stay in device memory. */
 int su2_dmrg_create(su2_ctx* ctx, int lx, int ly, double jcoup, double qcoup, int bdry,
                     int totspin, int max_dcut, su2_dmrg** out);
@@ -229,10 +255,90 @@ int su2_dmrg_energies(const su2_dmrg* d, double* out, int cap, int* n);
  * (lanczos_su2.cpp:237-338).  Any output may be NULL; *n receives the count. */
 int su2_dmrg_history(const su2_dmrg* d, int* kept, double* discarded, int* matvecs, int cap,
                      int* n);
-/* Timing counters: out[0..11] = environment s, Lanczos s, SVD s, projection
+/* Matvec arithmetic of later solves: bits = 64 (default, the reference's
+ * FP64 throughout) or 32 (mixed precision): the Lanczos Krylov matvecs run on
+ * FP32 MFMA with FP32 accumulation (su2_heff_set_precision), while the
+ * Ritz-vector check of lanczos_su2::check_eigenvector (lanczos_su2.cpp:
+ * 320-332) runs in FP64 and its Rayleigh quotient <psi|H psi> is the step's
+ * energy (an FP64 correction of the Ritz value, exact to second order in the
+ * Ritz vector's error).  Lanczos vectors and coefficients, the tridiagonal
+ * eigensolve, the SVD/truncation and the environment updates stay FP64.
+ * Solves whose matvec has fewer than min_flops algorithmic FLOPs stay FP64
+ * (min_flops < 0: the default 2e8, below which the FP32 copies cost more
+ * than they save).  A mixed run schedules 32 for the early sweeps and 64 for
+ * the final ones. */
+int su2_dmrg_set_matvec_precision(su2_ctx* ctx, su2_dmrg* d, int bits, double min_flops);
+/* H_eff launches of each solve replayed from a HIP graph (on = 1, default)
+ * or launched directly (0); the results are the same. */
+int su2_dmrg_set_graph(su2_ctx* ctx, su2_dmrg* d, int on);
+
+/* Several DMRG runs of one context advanced in lockstep (no counterpart in
+ * the reference, which runs one job per process).  Each tick takes the next
+ * two-site solve of every unfinished run (dmrg_su2_omp.cpp:20-157), runs all
+ * their Lanczos solves (lanczos_su2.cpp:125-195) together -- one grouped
+ * launch per GEMM stage for all of their H_eff applies each round, captured
+ * in a HIP graph and replayed, one host wait per round -- then one batched
+ * SVD over all their wavefunctions, then each run's environment update.
+ * Every problem keeps its own K split and every solve its own reduction
+ * order: each run's energies are bit for bit those it has alone. */
+int su2_batch_create(su2_ctx* ctx, su2_batch** out);
+int su2_batch_destroy(su2_ctx* ctx, su2_batch* b);
+/* Adds run d (same context, not owned; keep it until the batch is done with
+ * it): a warm-up, then nsweep sweeps with max_dcut[k] multiplets and matvec
+ * precision bits[k] (64 or 32 with min_fp32_flops as in
+ * su2_dmrg_set_matvec_precision; bits NULL: 64); the warm-up uses entry 0.
+ * stop > 0 ends the run once two successive sweeps differ by less than stop
+ * (main_omp.cpp:197).  *id: the run's index. */
+int su2_batch_add(su2_ctx* ctx, su2_batch* b, su2_dmrg* d, int nsweep, const int* max_dcut,
+                  const int* bits, double min_fp32_flops, double stop, int* id);
+/* One lockstep step; *stepped = number of runs that took a step (0: all done). */
+int su2_batch_tick(su2_ctx* ctx, su2_batch* b, int* stepped);
+/* Ticks until every run has finished. */
+int su2_batch_run(su2_ctx* ctx, su2_batch* b);
+int su2_batch_set_graph(su2_ctx* ctx, su2_batch* b, int on);
+/* Energy after each finished phase of run id (warm-up first); *finished. */
+int su2_batch_energies(const su2_batch* b, int id, int* finished, double* out, int cap, int* n);
+/* out[0..7] = ticks, HIP graphs built, graph replays, s preparing solves,
+ * s in Lanczos, device s of the batched applies, s in SVD, s completing. */
+int su2_batch_stats(const su2_batch* b, double* out, int cap);
+
+/* Request loop with continuous batching on one device (main_omp.cpp:42-202
+ * per job, dmrg_su2_omp.cpp:944-1068 for excited levels).  Jobs are admitted
+ * in arrival order once they have arrived, one of max_slots slots is free
+ * and their estimated device footprint (su2_job_footprint) fits what the
+ * admitted jobs leave of budget_bytes.  Admitted jobs keep their MPS and
+ * environments on the device and advance together, one solve each per
+ * su2_batch tick; a finished job frees its slot and memory at once and a job
+ * arriving meanwhile joins at the next tick.  An excited job of level k runs
+ * levels 0..k in its slot, saving each in the reference's file format under
+ * workdir/job{id} for the next (su2_dmrg_add_lower_state).  Each job's
+ * result is bit for bit what it gives served alone. */
+int su2_server_create(su2_ctx* ctx, double budget_bytes, int max_slots, const char* workdir,
+                      su2_server** out);
+int su2_server_destroy(su2_ctx* ctx, su2_server* s);
+/* arrival_s: seconds after the start of su2_server_run. */
+int su2_server_submit(su2_ctx* ctx, su2_server* s, const su2_job* job, double arrival_s,
+                      int* id);
+/* Serves every submitted job to completion (an idle device waits for the
+ * next arrival); verbose 1 logs admissions and completions on stderr. */
+int su2_server_run(su2_ctx* ctx, su2_server* s, int verbose);
+/* Final energy (Ritz value of the target level), times[3] = arrival, admission
+ * and completion in seconds from the start of the run, estimated footprint. */
+int su2_server_result(const su2_server* s, int id, double* energy, double* times,
+                      double* footprint);
+/* out[0..2] = wall seconds of the run, ticks, most jobs admitted at once. */
+int su2_server_stats(const su2_server* s, double* out, int cap);
+/* Estimated device bytes of a job (environments of every site, a step's
+ * H_eff bundles and Lanczos vectors, SVD work space; more for excited
+ * levels, which hold their lower states). */
+int su2_job_footprint(const su2_job* job, double* bytes);
+
+/* Timing counters: out[0..15] = environment s, Lanczos s, SVD s, projection
  * s, start-vector s, device matvec s, algorithmic H_eff FLOPs, matvecs, steps,
  * last centre bond dimension, s inside the one-sided Jacobi SVD, its sweeps
- * summed over solves.  cap bounds what is written. */
+ * summed over solves, FP32 matvecs, their device s, their algorithmic FLOPs,
+ * the largest |Ritz value - FP64 Rayleigh quotient| of a mixed solve.  cap
+ * bounds what is written. */
 int su2_dmrg_stats(const su2_dmrg* d, double* out, int cap);
 
 #ifdef __cplusplus

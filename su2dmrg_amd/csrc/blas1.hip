@@ -19,6 +19,64 @@ __global__ void fill_kernel(double* x, long long n, double v) {
   for (; i < n; i += stride) x[i] = v;
 }
 
+__global__ void fill_f32_kernel(float* x, long long n, float v) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) x[i] = v;
+}
+
+// rounding to nearest (the hardware conversion) and exact widening
+__global__ void f64_to_f32_kernel(float* __restrict__ y, const double* __restrict__ x,
+                                  long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) y[i] = (float)x[i];
+}
+
+__global__ void f32_to_f64_kernel(double* __restrict__ y, const float* __restrict__ x,
+                                  long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) y[i] = (double)x[i];
+}
+
+// One launch for many vector moves: descriptor blockIdx.y, a grid-stride loop
+// over its elements (a copy, a rounding to FP32, a widening to FP64, or a
+// fill with zero).
+__global__ void move_batch_kernel(const VecMove* __restrict__ desc) {
+  const VecMove d = desc[blockIdx.y];
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  switch (d.mode) {
+    case kMoveF64:
+      for (; i < d.n; i += stride) ((double*)d.dst)[i] = ((const double*)d.src)[i];
+      break;
+    case kMoveF64toF32:
+      for (; i < d.n; i += stride) ((float*)d.dst)[i] = (float)((const double*)d.src)[i];
+      break;
+    case kMoveF32toF64:
+      for (; i < d.n; i += stride) ((double*)d.dst)[i] = (double)((const float*)d.src)[i];
+      break;
+    case kZeroF64:
+      for (; i < d.n; i += stride) ((double*)d.dst)[i] = 0.0;
+      break;
+    case kZeroF32:
+      for (; i < d.n; i += stride) ((float*)d.dst)[i] = 0.0f;
+      break;
+    default:
+      break;
+  }
+}
+
+// x *= 1 / sqrt(s[0]) with s read on the device (normalisation without a
+// host round trip)
+__global__ void scal_inv_sqrt_kernel(double* x, long long n, const double* __restrict__ s) {
+  const double a = 1.0 / sqrt(s[0]);
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) x[i] *= a;
+}
+
 __global__ void copy_kernel(double* __restrict__ y, const double* __restrict__ x, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long stride = (long long)gridDim.x * blockDim.x;
@@ -228,9 +286,46 @@ unsigned grid_for(long long n) {
 
 }  // namespace
 
+
 void dev_fill(Context& c, double* x, long long n, double v) {
   if (n <= 0) return;
   hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(kThreads), 0, c.stream(), x, n, v);
+  SU2_HIP(hipGetLastError());
+}
+
+void dev_move_batch(Context& c, const VecMove* d_desc, int ndesc, long long max_n) {
+  if (ndesc <= 0 || max_n <= 0) return;
+  SU2_CHECK(ndesc < 65536, kErrCapacity, "vector moves: %d descriptors", ndesc);
+  unsigned gx = grid_for(max_n);
+  if (gx > 1024) gx = 1024;
+  hipLaunchKernelGGL(move_batch_kernel, dim3(gx, ndesc), dim3(kThreads), 0, c.stream(), d_desc);
+  SU2_HIP(hipGetLastError());
+}
+
+void dev_scal_inv_sqrt(Context& c, double* x, long long n, const double* d_s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(scal_inv_sqrt_kernel, dim3(grid_for(n)), dim3(kThreads), 0, c.stream(), x, n,
+                     d_s);
+  SU2_HIP(hipGetLastError());
+}
+
+void dev_fill_f32(Context& c, float* x, long long n, float v) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_f32_kernel, dim3(grid_for(n)), dim3(kThreads), 0, c.stream(), x, n, v);
+  SU2_HIP(hipGetLastError());
+}
+
+void dev_convert_f64_f32(Context& c, float* y, const double* x, long long n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(f64_to_f32_kernel, dim3(grid_for(n)), dim3(kThreads), 0, c.stream(), y, x,
+                     n);
+  SU2_HIP(hipGetLastError());
+}
+
+void dev_convert_f32_f64(Context& c, double* y, const float* x, long long n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(f32_to_f64_kernel, dim3(grid_for(n)), dim3(kThreads), 0, c.stream(), y, x,
+                     n);
   SU2_HIP(hipGetLastError());
 }
 

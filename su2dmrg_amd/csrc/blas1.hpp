@@ -16,6 +16,25 @@ class Context;
 
 void dev_fill(Context& c, double* x, long long n, double v);
 void dev_copy(Context& c, double* y, const double* x, long long n);
+// Many vector moves in one launch (descriptors in device memory, ndesc of
+// them, the longest max_n elements): copies, FP64 -> FP32 rounding, FP32 ->
+// FP64 widening, zero fills.  Used by the batched H_eff apply, whose launch
+// sequence is captured in a HIP graph (a fill kernel, never a memset node).
+enum VecMoveMode : int { kMoveF64 = 0, kMoveF64toF32 = 1, kMoveF32toF64 = 2, kZeroF64 = 3,
+                         kZeroF32 = 4 };
+struct VecMove {
+  const void* src;
+  void* dst;
+  long long n;
+  int mode, pad;
+};
+void dev_move_batch(Context& c, const VecMove* d_desc, int ndesc, long long max_n);
+// x *= 1 / sqrt(s[0]), s in device memory (no wait)
+void dev_scal_inv_sqrt(Context& c, double* x, long long n, const double* d_s);
+// FP32 buffers of the mixed-precision matvec: fill, round to FP32, widen
+void dev_fill_f32(Context& c, float* x, long long n, float v);
+void dev_convert_f64_f32(Context& c, float* y, const double* x, long long n);
+void dev_convert_f32_f64(Context& c, double* y, const float* x, long long n);
 void dev_scal(Context& c, double* x, long long n, double a);
 void dev_axpy(Context& c, double* y, const double* x, long long n, double a);       // y += a x
 void dev_axpby(Context& c, double* y, const double* x, long long n, double a, double b);  // y = a x + b y

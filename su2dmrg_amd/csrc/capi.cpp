@@ -14,6 +14,8 @@
 #include "dmrg.hpp"
 #include "heff.hpp"
 #include "recoupling.hpp"
+#include "batch.hpp"
+#include "server.hpp"
 #include "svd.hpp"
 #include "tensor.hpp"
 
@@ -35,6 +37,15 @@ struct su2_heff {
   su2::HeffPlan p;
   template <class... A>
   explicit su2_heff(A&&... a) : p(std::forward<A>(a)...) {}
+};
+struct su2_batch {
+  su2::BatchDriver b;
+  explicit su2_batch(su2::Context& c) : b(c) {}
+};
+struct su2_server {
+  su2::Server s;
+  template <class... A>
+  explicit su2_server(A&&... a) : s(std::forward<A>(a)...) {}
 };
 
 namespace {
@@ -79,7 +90,15 @@ int su2_ctx_create(int device, su2_ctx** out) {
 }
 
 int su2_ctx_destroy(su2_ctx* ctx) {
-  return guard(nullptr, [&] { delete ctx; });
+  if (!ctx) return SU2_OK;
+  // wait for the stream here, where a device error can be reported; the
+  // context is freed either way (its destructor does not throw)
+  const int rc = guard(nullptr, [&] {
+    ctx->c.make_current();
+    SU2_HIP(hipStreamSynchronize(ctx->c.stream()));
+  });
+  delete ctx;
+  return rc;
 }
 
 const char* su2_last_error(const su2_ctx* ctx) {
@@ -91,6 +110,18 @@ int su2_ctx_synchronize(su2_ctx* ctx) {
     need(ctx, "ctx");
     ctx->c.make_current();
     SU2_HIP(hipStreamSynchronize(ctx->c.stream()));
+  });
+}
+
+int su2_ctx_memory(su2_ctx* ctx, long long* live, long long* peak, long long* cached,
+                   int reset_peak) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    su2::DevicePool& p = *ctx->c.pool();
+    if (live) *live = (long long)p.live_bytes();
+    if (peak) *peak = (long long)p.peak_bytes();
+    if (cached) *cached = (long long)p.cached_bytes();
+    if (reset_peak) p.reset_peak();
   });
 }
 
@@ -348,9 +379,9 @@ int su2_heff_time(su2_ctx* ctx, su2_heff* plan, const su2_tensor* psi, su2_tenso
     for (int r = 0; r < reps; ++r) {
       float a = 0, b = 0;
       SU2_HIP(hipEventRecord(e0, s));
-      plan->p.stage1().launch(s);
+      plan->p.stage(0).launch(s);
       SU2_HIP(hipEventRecord(e2, s));
-      plan->p.stage2().launch(s);
+      plan->p.stage(1).launch(s);
       SU2_HIP(hipEventRecord(e3, s));
       SU2_HIP(hipEventSynchronize(e3));
       SU2_HIP(hipEventElapsedTime(&a, e0, e2));
@@ -380,7 +411,7 @@ int su2_heff_autotune(su2_ctx* ctx, su2_heff* plan, int reps, int* chosen, float
       chosen[0] = ch[0];
       chosen[1] = ch[1];
     }
-    const int nv = su2::gemm_num_variants();
+    const int nv = su2::gemm_num_variants(plan->p.precision());
     if (nvariants) *nvariants = nv;
     if (times) {
       for (int v = 0; v < nv; ++v) {
@@ -399,14 +430,23 @@ int su2_heff_set_variants(su2_ctx* ctx, su2_heff* plan, int stage1, int stage2) 
   });
 }
 
+int su2_heff_set_precision(su2_ctx* ctx, su2_heff* plan, int bits) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(plan, "plan");
+    SU2_CHECK(bits == 64 || bits == 32, su2::kErrArgument, "matvec precision %d bits", bits);
+    plan->p.set_precision(bits == 64 ? su2::Prec::F64 : su2::Prec::F32);
+  });
+}
+
 int su2_heff_set_kernel_path(su2_ctx* ctx, su2_heff* plan, int uniform, int* layouts) {
   return guard(ctx, [&] {
     need(ctx, "ctx");
     need(plan, "plan");
     plan->p.set_uniform(uniform != 0);
     if (layouts) {
-      layouts[0] = plan->p.stage1().layout();
-      layouts[1] = plan->p.stage2().layout();
+      layouts[0] = plan->p.stage(0).layout();
+      layouts[1] = plan->p.stage(1).layout();
     }
   });
 }
@@ -568,17 +608,206 @@ int su2_dmrg_history(const su2_dmrg* d, int* kept, double* discarded, int* matve
   });
 }
 
+int su2_dmrg_set_matvec_precision(su2_ctx* ctx, su2_dmrg* d, int bits, double min_flops) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(d, "dmrg");
+    SU2_CHECK(bits == 64 || bits == 32, su2::kErrArgument, "matvec precision %d bits", bits);
+    d->d.set_matvec_precision(bits == 64 ? su2::Prec::F64 : su2::Prec::F32, min_flops);
+  });
+}
+
+int su2_batch_create(su2_ctx* ctx, su2_batch** out) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(out, "out");
+    *out = new su2_batch(ctx->c);
+  });
+}
+
+int su2_batch_destroy(su2_ctx* ctx, su2_batch* b) {
+  return guard(ctx, [&] { delete b; });
+}
+
+int su2_batch_add(su2_ctx* ctx, su2_batch* b, su2_dmrg* d, int nsweep, const int* max_dcut,
+                  const int* bits, double min_fp32_flops, double stop, int* id) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(b, "batch");
+    need(d, "dmrg");
+    SU2_CHECK(nsweep >= 0, su2::kErrArgument, "batch: %d sweeps", nsweep);
+    std::vector<su2::RunPhase> prog;
+    su2::RunPhase w;
+    w.warmup = true;
+    w.max_dcut = nsweep > 0 && max_dcut ? max_dcut[0] : 0;
+    w.prec = nsweep > 0 && bits && bits[0] == 32 ? su2::Prec::F32 : su2::Prec::F64;
+    w.min_fp32_flops = min_fp32_flops;
+    prog.push_back(w);
+    for (int k = 0; k < nsweep; ++k) {
+      su2::RunPhase p;
+      p.max_dcut = max_dcut ? max_dcut[k] : 0;
+      SU2_CHECK(!bits || bits[k] == 64 || bits[k] == 32, su2::kErrArgument,
+                "batch: matvec precision %d bits", bits ? bits[k] : 0);
+      p.prec = bits && bits[k] == 32 ? su2::Prec::F32 : su2::Prec::F64;
+      p.min_fp32_flops = min_fp32_flops;
+      prog.push_back(p);
+    }
+    const int i = b->b.add(&d->d, prog, stop);
+    if (id) *id = i;
+  });
+}
+
+int su2_batch_tick(su2_ctx* ctx, su2_batch* b, int* stepped) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(b, "batch");
+    const int n = b->b.tick();
+    if (stepped) *stepped = n;
+  });
+}
+
+int su2_batch_run(su2_ctx* ctx, su2_batch* b) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(b, "batch");
+    while (b->b.tick() > 0) {
+    }
+  });
+}
+
+int su2_batch_set_graph(su2_ctx* ctx, su2_batch* b, int on) {
+  return guard(ctx, [&] {
+    need(b, "batch");
+    b->b.set_use_graph(on != 0);
+  });
+}
+
+int su2_batch_energies(const su2_batch* b, int id, int* finished, double* out, int cap, int* n) {
+  return guard(nullptr, [&] {
+    need(b, "batch");
+    SU2_CHECK(id >= 0, su2::kErrArgument, "batch: run %d", id);
+    const std::vector<double>& e = b->b.phase_energies(id);
+    if (finished) *finished = b->b.finished(id) ? 1 : 0;
+    for (int i = 0; i < (int)e.size() && i < cap && out; ++i) out[i] = e[i];
+    if (n) *n = (int)e.size();
+  });
+}
+
+int su2_batch_stats(const su2_batch* b, double* out, int cap) {
+  return guard(nullptr, [&] {
+    need(b, "batch");
+    need(out, "out");
+    const su2::BatchDriver& d = b->b;
+    const double v[8] = {(double)d.ticks(),   (double)d.graphs_built(), (double)d.graph_replays(),
+                         d.t_prepare,         d.t_lanczos,              d.t_matvec,
+                         d.t_svd,             d.t_complete};
+    for (int i = 0; i < 8 && i < cap; ++i) out[i] = v[i];
+  });
+}
+
+int su2_dmrg_set_graph(su2_ctx* ctx, su2_dmrg* d, int on) {
+  return guard(ctx, [&] {
+    need(d, "dmrg");
+    d->d.set_use_graph(on != 0);
+  });
+}
+
+int su2_server_create(su2_ctx* ctx, double budget_bytes, int max_slots, const char* workdir,
+                      su2_server** out) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(out, "out");
+    need(workdir, "workdir");
+    *out = new su2_server(ctx->c, budget_bytes, max_slots, std::string(workdir));
+  });
+}
+
+int su2_server_destroy(su2_ctx* ctx, su2_server* s) {
+  return guard(ctx, [&] { delete s; });
+}
+
+int su2_server_submit(su2_ctx* ctx, su2_server* s, const su2_job* job, double arrival_s,
+                      int* id) {
+  return guard(ctx, [&] {
+    need(s, "server");
+    need(job, "job");
+    su2::JobSpec j;
+    j.lx = job->lx;
+    j.ly = job->ly;
+    j.jcoup = job->jcoup;
+    j.qcoup = job->qcoup;
+    j.bdry = job->bdry;
+    j.totspin = job->totspin;
+    j.max_dcut = job->max_dcut;
+    j.exci = job->exci;
+    j.nsweep = job->nsweep;
+    j.stop = job->stop;
+    const int i = s->s.submit(j, arrival_s);
+    if (id) *id = i;
+  });
+}
+
+int su2_server_run(su2_ctx* ctx, su2_server* s, int verbose) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(s, "server");
+    s->s.set_verbose(verbose);
+    s->s.run();
+  });
+}
+
+int su2_server_result(const su2_server* s, int id, double* energy, double* times,
+                      double* footprint) {
+  return guard(nullptr, [&] {
+    need(s, "server");
+    SU2_CHECK(id >= 0 && id < s->s.njobs(), su2::kErrArgument, "server: job %d", id);
+    const su2::JobResult& r = s->s.result(id);
+    SU2_CHECK(r.done, su2::kErrArgument, "server: job %d has not finished", id);
+    if (energy) *energy = r.energy;
+    if (times) {
+      times[0] = r.t_arrive;
+      times[1] = r.t_admit;
+      times[2] = r.t_finish;
+    }
+    if (footprint) *footprint = r.footprint;
+  });
+}
+
+int su2_server_stats(const su2_server* s, double* out, int cap) {
+  return guard(nullptr, [&] {
+    need(s, "server");
+    need(out, "out");
+    const double v[3] = {s->s.wall(), (double)s->s.ticks(), (double)s->s.max_concurrent()};
+    for (int i = 0; i < 3 && i < cap; ++i) out[i] = v[i];
+  });
+}
+
+int su2_job_footprint(const su2_job* job, double* bytes) {
+  return guard(nullptr, [&] {
+    need(job, "job");
+    need(bytes, "bytes");
+    su2::JobSpec j;
+    j.lx = job->lx;
+    j.ly = job->ly;
+    j.qcoup = job->qcoup;
+    j.max_dcut = job->max_dcut;
+    j.exci = job->exci;
+    *bytes = su2::job_footprint(j);
+  });
+}
+
 int su2_dmrg_stats(const su2_dmrg* d, double* out, int cap) {
   return guard(nullptr, [&] {
     need(d, "dmrg");
     need(out, "out");
     const su2::DmrgStats& s = d->d.stats();
-    const double v[12] = {s.t_env,      s.t_lanczos,         s.t_svd,
+    const double v[16] = {s.t_env,      s.t_lanczos,         s.t_svd,
                           s.t_project,  s.t_start,           s.t_matvec,
                           s.heff_flops, (double)s.matvecs,   (double)s.steps,
                           (double)d->d.last_bond_dim(), s.t_jacobi,
-                          (double)s.jacobi_sweeps};
-    for (int i = 0; i < 12 && i < cap; ++i) out[i] = v[i];
+                          (double)s.jacobi_sweeps, (double)s.matvecs32, s.t_matvec32,
+                          s.heff_flops32, s.max_fp64_correction};
+    for (int i = 0; i < 16 && i < cap; ++i) out[i] = v[i];
   });
 }
 

@@ -34,7 +34,12 @@ class DevicePool {
   void* alloc(size_t bytes);   // alloc(0) is nullptr
   void free(void* p);
   void set_stream(hipStream_t s);
-  void detach();
+  void detach() noexcept;
+  // bytes handed out now, the most handed out at once, bytes held by the cache
+  size_t live_bytes() const { return live_bytes_; }
+  size_t peak_bytes() const { return peak_bytes_; }
+  size_t cached_bytes() const;
+  void reset_peak() { peak_bytes_ = live_bytes_; }
 
  private:
   void recycle();        // synchronise the stream, held-back blocks become free
@@ -45,6 +50,7 @@ class DevicePool {
   std::multimap<size_t, void*> free_;           // size -> block, reusable now
   std::vector<std::pair<size_t, void*>> held_;  // freed since the last sync
   std::unordered_map<void*, size_t> live_;      // handed out
+  size_t live_bytes_ = 0, peak_bytes_ = 0;
 };
 
 class Context {
@@ -88,6 +94,30 @@ class Context {
   size_t d_red_n_ = 0;
   double* h_red_ = nullptr;
   size_t h_red_n_ = 0;
+};
+
+// Device buffer of doubles from a context's cache, returned to it when the
+// holder goes out of scope (also when an exception unwinds the scope).
+struct DevBuf {
+  Context* c = nullptr;
+  double* p = nullptr;
+  DevBuf() = default;
+  DevBuf(Context& ctx, long long n) : c(&ctx), p(ctx.dev_alloc_n<double>(n > 0 ? n : 0)) {}
+  ~DevBuf() {
+    if (c) c->dev_free(p);
+  }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : c(o.c), p(o.p) { o.p = nullptr; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      if (c) c->dev_free(p);
+      c = o.c;
+      p = o.p;
+      o.p = nullptr;
+    }
+    return *this;
+  }
 };
 
 }  // namespace su2

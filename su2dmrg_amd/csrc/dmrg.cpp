@@ -8,6 +8,7 @@
 #include "blas1.hpp"
 #include "common.hpp"
 #include "context.hpp"
+#include "batch.hpp"
 #include "heff.hpp"
 #include "lanczos.hpp"
 #include "ops.hpp"
@@ -315,32 +316,12 @@ std::unique_ptr<Env> Dmrg::project(const Tensor& P, const Env& hat) {
   return out;
 }
 
-Dmrg::StepOut Dmrg::step(int il, int ir, std::unique_ptr<Tensor> psi0, double p1, double p2) {
-  Context& c = c_;
-  StepOut o;
-  const double t_step = now();
-  double t0 = t_step;
-  o.FL = fused_left(il);
-  o.FR = fused_right(ir);
-  o.El = enlarged_left(il, o.FL);
-  o.Er = enlarged_right(ir, o.FR);
-  stats_.t_env += now() - t0;
+Dmrg::Solve::~Solve() = default;
 
-  t0 = now();
-  std::unique_ptr<Tensor> psi(
-      new Tensor(&c, {with_dir(o.FL.fused, 1), with_dir(o.FR.fused, 1)}, 0, nullptr));
-  SU2_CHECK(psi->size() > 0, kErrStructure, "step %d,%d: no common sector for spin %d", il, ir,
-            S_);
-  if (psi0) {
-    SU2_CHECK(psi0->size() == psi->size(), kErrStructure, "start vector layout mismatch");
-    dev_copy(c, psi->data(), psi0->data(), psi->size());
-  } else {
-    dev_fill(c, psi->data(), psi->size(), 1.0);   // initialize_input_vector (tensor_su2.cpp:2354-2372)
-  }
-  // H_eff terms (dmrg_su2.cpp:571-610): couplings summed once per solve
+// H_eff terms of the cut (il, ir) (dmrg_su2.cpp:571-610): couplings summed
+// once per solve, the plaquette terms appended
+std::vector<HeffTerm> Dmrg::heff_terms(int il, int ir, const Env& El, const Env& Er) const {
   std::vector<HeffTerm> terms;
-  const Env& El = *o.El;
-  const Env& Er = *o.Er;
   if (il < lat_.ns / 2) {
     for (int j = 0; j <= il; ++j) {
       auto lj = El.S.find(j);
@@ -365,59 +346,150 @@ Dmrg::StepOut Dmrg::step(int il, int ir, std::unique_ptr<Tensor> psi0, double p1
     }
   }
   plaquette_terms(il, ir, El, Er, terms);
-  HeffPlan plan(c, *psi, El.H.get(), Er.H.get(), terms);
-  if (var1_ >= 0) plan.set_variants(var1_, var2_);
-  const int mlanc = (int)std::min<long long>(kMlanc, psi->size() + 1);
+  return terms;
+}
+
+// Everything of a two-site step before its Lanczos run: enlarged
+// environments, the start vector in the two-site basis, the H_eff plan and
+// the lower states (excited-state penalty).
+std::unique_ptr<Dmrg::Solve> Dmrg::prepare(int il, int ir, std::unique_ptr<Tensor> psi0, double p1,
+                                           double p2) {
+  Context& c = c_;
+  std::unique_ptr<Solve> sv(new Solve());
+  Solve& S = *sv;
+  S.il = il;
+  S.ir = ir;
+  S.p1 = p1;
+  S.p2 = p2;
+  S.t_step = now();
+  double t0 = S.t_step;
+  StepOut& o = S.o;
+  o.FL = fused_left(il);
+  o.FR = fused_right(ir);
+  o.El = enlarged_left(il, o.FL);
+  o.Er = enlarged_right(ir, o.FR);
+  stats_.t_env += now() - t0;
+
+  t0 = now();
+  S.t_lanczos0 = t0;
+  S.psi.reset(new Tensor(&c, {with_dir(o.FL.fused, 1), with_dir(o.FR.fused, 1)}, 0, nullptr));
+  SU2_CHECK(S.psi->size() > 0, kErrStructure, "step %d,%d: no common sector for spin %d", il, ir,
+            S_);
+  if (psi0) {
+    SU2_CHECK(psi0->size() == S.psi->size(), kErrStructure, "start vector layout mismatch");
+    dev_copy(c, S.psi->data(), psi0->data(), S.psi->size());
+  } else {
+    dev_fill(c, S.psi->data(), S.psi->size(), 1.0);   // initialize_input_vector (tensor_su2.cpp:2354-2372)
+  }
+  std::vector<HeffTerm> terms = heff_terms(il, ir, *o.El, *o.Er);
+  S.plan.reset(new HeffPlan(c, *S.psi, o.El->H.get(), o.Er->H.get(), terms));
+  if (var1_ >= 0) S.plan->set_variants(var1_, var2_);
+  // mixed precision: FP32 Krylov matvecs where the matvec is large enough
+  // to gain from them; the Ritz-vector checks stay FP64
+  S.fp32 = prec_ == Prec::F32 && S.plan->algorithmic_flops() >= min_fp32_flops_;
+  if (S.fp32) {
+    S.plan->set_precision(Prec::F32);
+    if (var32_[0] >= 0) S.plan->set_variants(var32_[0], var32_[1]);
+    S.plan->set_precision(Prec::F64);
+  }
+  S.mlanc = (int)std::min<long long>(kMlanc, S.psi->size() + 1);
   // lower states in this two-site basis (adjacent sites only, dmrg_su2.cpp:611)
-  std::vector<std::unique_ptr<Tensor>> phis;
-  if (!lower_.empty() && il + 1 == ir) phis = lower_vectors(il, ir, o.FL, o.FR, *psi);
+  if (!lower_.empty() && il + 1 == ir) {
+    S.phis = lower_vectors(il, ir, o.FL, o.FR, *S.psi);
+    S.pen = DevBuf(c, 1);
+  }
+  return sv;
+}
+
+// - sum_j E_j <phi_j|x> phi_j (dmrg_su2.cpp:611-623), the overlaps kept on
+// the device (no host wait)
+void Dmrg::penalty(const Solve& S, const double* x, double* y) {
+  for (size_t j = 0; j < S.phis.size(); ++j) {
+    dev_dot_to(c_, S.phis[j]->data(), x, S.psi->size(), S.pen.p);
+    dev_axpy_dev(c_, y, S.phis[j]->data(), S.psi->size(), S.pen.p, -lower_[j]->energy);
+  }
+}
+
+// The solve's Lanczos run, as a batch of one (BatchDriver runs the same
+// operations for many solves at once).
+LanczosResult Dmrg::solve(Solve& S) {
+  Context& c = c_;
+  HeffBatch hb(c);
+  hb.set_use_graph(use_graph_);
+  std::vector<LanczosJob> jobs(1);
+  jobs[0].psi = S.psi->data();
+  jobs[0].n = S.psi->size();
+  jobs[0].mlanc = S.mlanc;
+  jobs[0].mixed = S.fp32;
   // matvec device time from an event pair per matvec, read once the solve
-  // has finished (no wait per matvec)
-  std::vector<hipEvent_t> ev;
+  // has finished (no wait per matvec); FP32 and FP64 matvecs apart
+  std::vector<std::pair<hipEvent_t, int>> ev;
   auto release_events = [&]() {
-    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    for (auto& e : ev) (void)hipEventDestroy(e.first);
     ev.clear();
   };
-  float mv_ms = 0;
-  auto apply = [&](const double* x, double* y) {
-    hipEvent_t e0, e1;
-    SU2_HIP(hipEventCreate(&e0));
-    ev.push_back(e0);
-    SU2_HIP(hipEventCreate(&e1));
-    ev.push_back(e1);
-    SU2_HIP(hipEventRecord(e0, c.stream()));
-    plan.apply_raw(x, y);
-    SU2_HIP(hipEventRecord(e1, c.stream()));
-    // - sum_j E_j <phi_j|x> phi_j (dmrg_su2.cpp:611-623)
-    for (size_t j = 0; j < phis.size(); ++j) {
-      const double ov = dev_dot(c, phis[j]->data(), x, psi->size());
-      dev_axpy(c, y, phis[j]->data(), psi->size(), -lower_[j]->energy * ov);
+  auto apply = [&](const std::vector<LanczosReq>& reqs) {
+    for (auto& r : reqs) {
+      const Prec p = (S.fp32 && !r.check) ? Prec::F32 : Prec::F64;
+      hipEvent_t e0, e1;
+      SU2_HIP(hipEventCreate(&e0));
+      ev.push_back({e0, (int)p});
+      SU2_HIP(hipEventCreate(&e1));
+      ev.push_back({e1, (int)p});
+      SU2_HIP(hipEventRecord(e0, c.stream()));
+      hb.apply({HeffBatch::Req{S.plan.get(), p, r.x, r.y}});
+      SU2_HIP(hipEventRecord(e1, c.stream()));
+      penalty(S, r.x, r.y);
     }
   };
-  LanczosResult lr;
+  double ms[2] = {0, 0};
   try {
-    lr = lanczos_ground(c, apply, psi->data(), psi->size(), mlanc);
+    lanczos_batch(c, jobs, apply);
     SU2_HIP(hipStreamSynchronize(c.stream()));
     for (size_t i = 0; i + 1 < ev.size(); i += 2) {
-      float ms = 0;
-      SU2_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
-      mv_ms += ms;
+      float t = 0;
+      SU2_HIP(hipEventElapsedTime(&t, ev[i].first, ev[i + 1].first));
+      ms[ev[i].second == (int)Prec::F32 ? 1 : 0] += t;
     }
   } catch (...) {
     release_events();
     throw;
   }
   release_events();
-  stats_.t_matvec += mv_ms * 1e-3;
+  hb.clear();
+  const LanczosResult& lr = jobs[0].res;
+  HeffPlan& plan = *S.plan;
+  const int n32 = S.fp32 ? lr.matvecs - lr.check_matvecs : 0;
+  stats_.t_matvec += (ms[0] + ms[1]) * 1e-3;
+  stats_.t_matvec32 += ms[1] * 1e-3;
   stats_.matvecs += lr.matvecs;
+  stats_.matvecs32 += n32;
   stats_.heff_flops += plan.algorithmic_flops() * lr.matvecs;
-  stats_.t_lanczos += now() - t0;
+  stats_.heff_flops32 += plan.algorithmic_flops() * n32;
+  if (S.fp32)
+    stats_.max_fp64_correction =
+        std::max(stats_.max_fp64_correction, std::fabs(lr.ritz - lr.energy));
+  stats_.t_lanczos += now() - S.t_lanczos0;
+  release_plan(S);
+  return jobs[0].res;
+}
 
-  t0 = now();
-  SvdResult sv = truncated_svd(c, *psi, p1, p2, D_);
+SvdResult Dmrg::svd(Solve& S) {
+  const double t0 = now();
+  SvdResult sv = truncated_svd(c_, *S.psi, S.p1, S.p2, D_);
   stats_.t_svd += now() - t0;
-  stats_.t_jacobi += sv.t_jacobi;
-  stats_.jacobi_sweeps += sv.jacobi_sweeps;
+  return sv;
+}
+
+// The plan's bundles go back to the cache before the SVD allocates.
+void Dmrg::release_plan(Solve& S) {
+  S.plan.reset();
+  S.phis.clear();
+}
+
+Dmrg::StepOut Dmrg::finish(Solve& S, const LanczosResult& lr, SvdResult&& sv) {
+  StepOut o = std::move(S.o);
+  o.tag = S.tag;
   o.energy = lr.energy;
   o.U = std::move(sv.U);
   o.V = std::move(sv.V);
@@ -426,10 +498,12 @@ Dmrg::StepOut Dmrg::step(int il, int ir, std::unique_ptr<Tensor> psi0, double p1
   history_.push_back({(int)sv.bond.total(), sv.discarded_weight, lr.matvecs});
   wout_ = sv.wout;
   last_bond_ = (int)sv.bond.total();
+  stats_.t_jacobi += sv.t_jacobi;
+  stats_.jacobi_sweeps += sv.jacobi_sweeps;
   if (verbose_) {
-    std::fprintf(stderr, "step %lld (%d,%d) E=%.12f kept=%d discarded=%.3e matvecs=%d %.3fs\n",
-                 (long long)energies_.size() - 1, il, ir, lr.energy, last_bond_,
-                 sv.discarded_weight, lr.matvecs, now() - t_step);
+    std::fprintf(stderr, "step %lld (%d,%d) E=%.12f kept=%d discarded=%.3e matvecs=%d%s %.3fs\n",
+                 (long long)energies_.size() - 1, S.il, S.ir, lr.energy, last_bond_,
+                 sv.discarded_weight, lr.matvecs, S.fp32 ? " fp32" : "", now() - S.t_step);
     std::fflush(stderr);
   }
   stats_.steps++;
@@ -578,109 +652,164 @@ void Dmrg::ensure_right(int i) {
   overlap_right(i);
 }
 
-// do_idmrg (dmrg_su2_omp.cpp:20-95)
-double Dmrg::warmup() {
+// The reference's step order as a cursor (dmrg_su2_omp.cpp:20-157), so that a
+// driver can prepare one solve of each of several runs, run their Lanczos
+// solves and SVDs together, and complete each.  warmup() and sweep() drive
+// one run through the same cursor.
+
+void Dmrg::begin_warmup() {
+  c_.make_current();
+  cur_ = Cursor{};
+  cur_.phase = kPhaseWarmup;
+  cur_.i = 0;
+}
+
+void Dmrg::begin_sweep() {
   c_.make_current();
   const int ns = lat_.ns;
-  double last = 0;
-  for (int i = 0; i <= ns / 2 - 2; ++i) {
-    ensure_left(i);
-    ensure_right(ns - 1 - i);
-    const int il = i + 1, ir = ns - 2 - i;
-    const bool final = (i == ns / 2 - 2);
-    const bool solve = ((i + 2) * 2 * kPhys >= S_ && i >= 1) || final;
-    if (!solve) continue;
-    StepOut o = step(il, ir, nullptr, final ? 1.0 : 0.0, 0.0);
-    lb_[il] = o.bond;
-    rb_[ir] = o.bond;
+  SU2_CHECK(U_.count(ns / 2 - 1), kErrArgument, "sweep before warm-up");
+  cur_ = Cursor{};
+  cur_.phase = kPhaseSweep;
+  cur_.A = std::move(U_[ns / 2 - 1]);
+  U_.erase(ns / 2 - 1);
+  cur_.bond = lb_.at(ns / 2 - 1);
+  cur_.energy = energies_.empty() ? 0 : energies_.back();
+  cur_.pass = 1;
+  cur_.i = ns / 2;
+}
+
+std::unique_ptr<Dmrg::Solve> Dmrg::next_solve() {
+  const int ns = lat_.ns;
+  const int nf = kNfree;
+  if (cur_.phase == kPhaseWarmup) {
+    // do_idmrg (dmrg_su2_omp.cpp:20-95)
+    for (; cur_.i <= ns / 2 - 2; ++cur_.i) {
+      const int i = cur_.i;
+      ensure_left(i);
+      ensure_right(ns - 1 - i);
+      const bool final = (i == ns / 2 - 2);
+      const bool solve = ((i + 2) * 2 * kPhys >= S_ && i >= 1) || final;
+      if (!solve) continue;
+      auto S = prepare(i + 1, ns - 2 - i, nullptr, final ? 1.0 : 0.0, 0.0);
+      S->tag = final ? kTagWarmFinal : kTagWarm;
+      return S;
+    }
+    cur_.phase = kPhaseNone;
+    return nullptr;
+  }
+  if (cur_.phase != kPhaseSweep) return nullptr;
+  // dmrg_su2::sweep (dmrg_su2_omp.cpp:98-157): pass 1 centre -> left edge,
+  // pass 2 left edge -> right edge, pass 3 right edge -> centre
+  if (cur_.pass == 1 && cur_.i < nf + 2) {
+    cur_.pass = 2;
+    cur_.i = nf;
+  }
+  if (cur_.pass == 2 && cur_.i >= ns - nf - 2) {
+    cur_.pass = 3;
+    cur_.i = ns - nf - 1;
+  }
+  if (cur_.pass == 3 && cur_.i <= ns / 2) {
+    // end of the sweep: the centre's left factor carries the state
+    if (cur_.il_last < 0) cur_.il_last = ns / 2 - 1;
+    U_[cur_.il_last] = std::move(cur_.A);
+    cur_.phase = kPhaseNone;
+    return nullptr;
+  }
+  const int i = cur_.i;
+  std::unique_ptr<Solve> S;
+  if (cur_.pass == 1) {
+    const int il = i - 2, ir = i - 1;
+    const bool turn = (i == nf + 2);
+    S = prepare(il, ir, start_left(il, ir, *cur_.A, cur_.bond), turn ? 0.0 : 1.0,
+                turn ? 1.0 : 0.0);
+    S->tag = turn ? kTagP1Turn : kTagP1;
+  } else if (cur_.pass == 2) {
+    const int il = i + 1, ir = i + 2;
+    const bool turn = (i == ns - nf - 3);
+    S = prepare(il, ir, start_right(il, ir, *cur_.B, cur_.bond), turn ? 1.0 : 0.0,
+                turn ? 0.0 : 1.0);
+    S->tag = turn ? kTagP2Turn : kTagP2;
+  } else {
+    const int il = i - 2, ir = i - 1;
+    S = prepare(il, ir, start_left(il, ir, *cur_.A, cur_.bond), 1.0, 0.0);
+    S->tag = kTagP3;
+  }
+  return S;
+}
+
+void Dmrg::complete(std::unique_ptr<Solve> S, const LanczosResult& lr, SvdResult&& sv) {
+  StepOut o = finish(*S, lr, std::move(sv));
+  const int il = S->il, ir = S->ir;
+  S.reset();
+  lb_[il] = o.bond;
+  rb_[ir] = o.bond;
+  auto right_env = [&]() {
     R_[ir] = project(*o.V, *o.Er);
     V_[ir] = std::move(o.V);
     overlap_right(ir);
-    if (final) {
+  };
+  auto left_env = [&]() {
+    L_[il] = project(*o.U, *o.El);
+    U_[il] = std::move(o.U);
+    overlap_left(il);
+  };
+  switch (o.tag) {
+    case kTagWarm:
+      right_env();
+      left_env();
+      break;
+    case kTagWarmFinal:
+      right_env();
       U_[il] = std::move(o.U);
-    } else {
-      L_[il] = project(*o.U, *o.El);
-      U_[il] = std::move(o.U);
-      overlap_left(il);
-    }
-    last = o.energy;
+      break;
+    case kTagP1:    // centre -> left: the right block grows, A carries on
+    case kTagP2Turn:
+    case kTagP3:
+      right_env();
+      cur_.A = std::move(o.U);
+      if (o.tag == kTagP3) cur_.il_last = il;
+      break;
+    case kTagP1Turn:  // turn at the left edge: B carries on to the right
+    case kTagP2:
+      left_env();
+      cur_.B = std::move(o.V);
+      break;
+    default:
+      fail(kErrArgument, "dmrg: solve without a step tag");
+  }
+  cur_.bond = lb_.at(il);
+  cur_.energy = o.energy;
+  if (cur_.phase == kPhaseWarmup) {
+    ++cur_.i;
+  } else if (cur_.pass == 1 || cur_.pass == 3) {
+    --cur_.i;
+  } else {
+    ++cur_.i;
+  }
+}
+
+// do_idmrg (dmrg_su2_omp.cpp:20-95)
+double Dmrg::warmup() {
+  begin_warmup();
+  double last = 0;
+  while (std::unique_ptr<Solve> S = next_solve()) {
+    LanczosResult lr = solve(*S);
+    SvdResult sv = svd(*S);
+    last = lr.energy;
+    complete(std::move(S), lr, std::move(sv));
   }
   return last;
 }
 
 // dmrg_su2::sweep (dmrg_su2_omp.cpp:98-157)
 double Dmrg::sweep() {
-  c_.make_current();
-  const int ns = lat_.ns;
-  const int nf = kNfree;
-  SU2_CHECK(U_.count(ns / 2 - 1), kErrArgument, "sweep before warm-up");
-  std::unique_ptr<Tensor> A = std::move(U_[ns / 2 - 1]), B;
-  U_.erase(ns / 2 - 1);
-  Leg bond = lb_.at(ns / 2 - 1);
-  double e = energies_.empty() ? 0 : energies_.back();
-  // pass 1: centre -> left edge
-  for (int i = ns / 2; i >= nf + 2; --i) {
-    const int il = i - 2, ir = i - 1;
-    std::unique_ptr<Tensor> psi0 = start_left(il, ir, *A, bond);
-    const bool turn = (i == nf + 2);
-    StepOut o = step(il, ir, std::move(psi0), turn ? 0.0 : 1.0, turn ? 1.0 : 0.0);
-    bond = o.bond;
-    lb_[il] = bond;
-    rb_[ir] = bond;
-    if (!turn) {
-      R_[ir] = project(*o.V, *o.Er);
-      V_[ir] = std::move(o.V);
-      overlap_right(ir);
-      A = std::move(o.U);
-    } else {
-      L_[il] = project(*o.U, *o.El);
-      U_[il] = std::move(o.U);
-      overlap_left(il);
-      B = std::move(o.V);
-    }
-    e = o.energy;
+  begin_sweep();
+  while (std::unique_ptr<Solve> S = next_solve()) {
+    LanczosResult lr = solve(*S);
+    SvdResult sv = svd(*S);
+    complete(std::move(S), lr, std::move(sv));
   }
-  // pass 2: left edge -> right edge
-  for (int i = nf; i < ns - nf - 2; ++i) {
-    const int il = i + 1, ir = i + 2;
-    std::unique_ptr<Tensor> psi0 = start_right(il, ir, *B, bond);
-    const bool turn = (i == ns - nf - 3);
-    StepOut o = step(il, ir, std::move(psi0), turn ? 1.0 : 0.0, turn ? 0.0 : 1.0);
-    bond = o.bond;
-    lb_[il] = bond;
-    rb_[ir] = bond;
-    if (!turn) {
-      L_[il] = project(*o.U, *o.El);
-      U_[il] = std::move(o.U);
-      overlap_left(il);
-      B = std::move(o.V);
-    } else {
-      R_[ir] = project(*o.V, *o.Er);
-      V_[ir] = std::move(o.V);
-      overlap_right(ir);
-      A = std::move(o.U);
-    }
-    e = o.energy;
-  }
-  // pass 3: right edge -> centre
-  int il_last = -1;
-  for (int i = ns - nf - 1; i > ns / 2; --i) {
-    const int il = i - 2, ir = i - 1;
-    std::unique_ptr<Tensor> psi0 = start_left(il, ir, *A, bond);
-    StepOut o = step(il, ir, std::move(psi0), 1.0, 0.0);
-    bond = o.bond;
-    lb_[il] = bond;
-    rb_[ir] = bond;
-    R_[ir] = project(*o.V, *o.Er);
-    V_[ir] = std::move(o.V);
-    overlap_right(ir);
-    A = std::move(o.U);
-    il_last = il;
-    e = o.energy;
-  }
-  if (il_last < 0) il_last = ns / 2 - 1;
-  U_[il_last] = std::move(A);
-  return e;
+  return cur_.energy;
 }
 
 }  // namespace su2

@@ -23,13 +23,18 @@
 #include <tuple>
 #include <vector>
 
+#include "context.hpp"
+#include "gemm.hpp"
+#include "lanczos.hpp"
 #include "lattice.hpp"
 #include "structure.hpp"
+#include "svd.hpp"
 #include "tensor.hpp"
 
 namespace su2 {
 
 class Context;
+class HeffPlan;
 struct HeffTerm;
 
 struct Env {
@@ -87,6 +92,12 @@ struct DmrgStats {
   double t_matvec = 0;         // device time inside H_eff applies (events)
   double t_jacobi = 0;         // time inside the one-sided Jacobi SVD
   long long jacobi_sweeps = 0; // its sweeps, summed over solves
+  // mixed precision: FP32 matvecs, their device time and algorithmic
+  // FLOPs, and the largest |Ritz value - FP64 Rayleigh quotient| of a solve
+  long long matvecs32 = 0;
+  double t_matvec32 = 0;
+  double heff_flops32 = 0;
+  double max_fp64_correction = 0;
 };
 
 class Dmrg {
@@ -111,6 +122,21 @@ class Dmrg {
   // warmup(), or after load() and before the next sweep.
   void add_lower_state(const std::string& dir, int file_dcut, int ky, int exci);
   int lower_states() const { return (int)lower_.size(); }
+
+  // Step cursor over the reference's step order (dmrg_su2_omp.cpp:20-157):
+  // begin a phase (the warm-up or one sweep), then take its two-site solves
+  // one at a time with next_solve() (nullptr: the phase is over), run each
+  // one's Lanczos (lanczos_ground / lanczos_batch with solve_spec) and SVD,
+  // and hand both back to complete().  warmup() and sweep() are this loop
+  // for one run; BatchDriver runs it for several runs in lockstep.
+  struct Solve;
+  void begin_warmup();
+  void begin_sweep();
+  std::unique_ptr<Solve> next_solve();
+  void complete(std::unique_ptr<Solve> S, const LanczosResult& lr, SvdResult&& sv);
+  double phase_energy() const { return cur_.energy; }
+  int max_dcut() const { return D_; }
+  Context& context() { return c_; }
   void set_max_dcut(int d) { D_ = d; }
   // 1: one line per solve on stderr (step, sites, energy, kept multiplets,
   // discarded weight, matvecs, seconds), as the reference logs each step
@@ -124,9 +150,31 @@ class Dmrg {
     var1_ = v1;
     var2_ = v2;
   }
+  void set_heff_variants32(int v1, int v2) {
+    var32_[0] = v1;
+    var32_[1] = v2;
+  }
+  // Matvec arithmetic of later solves: FP64, or mixed (FP32 MFMA Krylov
+  // matvecs, FP64 Ritz-vector checks whose Rayleigh quotient is the energy;
+  // Lanczos vectors, tridiagonal solve and SVD stay FP64)
+  void set_matvec_precision(Prec p, double min_flops = -1) {
+    prec_ = p;
+    min_fp32_flops_ = min_flops < 0 ? kDefaultMinFp32Flops : min_flops;
+  }
+  // smallest matvec (algorithmic FLOPs) that a mixed-precision run does in
+  // FP32: below it the FP32 copies cost more than the faster launches save
+  static constexpr double kDefaultMinFp32Flops = 2e8;
+  Prec matvec_precision() const { return prec_; }
+  // replay each solve's H_eff launches from a HIP graph (default) or launch
+  // them directly; the results are the same
+  void set_use_graph(bool on) { use_graph_ = on; }
 
  private:
+  // what completing a solve does with its factors (the branches of
+  // dmrg_su2_omp.cpp:20-157)
+  enum Tag { kTagNone, kTagWarm, kTagWarmFinal, kTagP1, kTagP1Turn, kTagP2, kTagP2Turn, kTagP3 };
   struct StepOut {
+    int tag = kTagNone;
     double energy = 0;
     std::unique_ptr<Tensor> U, V;
     Leg bond;
@@ -147,7 +195,45 @@ class Dmrg {
   void plaquette_terms(int il, int ir, const Env& El, const Env& Er,
                        std::vector<HeffTerm>& terms) const;
   std::unique_ptr<Env> project(const Tensor& P, const Env& hat);
-  StepOut step(int il, int ir, std::unique_ptr<Tensor> psi0, double p1, double p2);
+  enum Phase { kPhaseNone, kPhaseWarmup, kPhaseSweep };
+  struct Cursor {
+    int phase = kPhaseNone;
+    int pass = 0, i = 0, il_last = -1;
+    std::unique_ptr<Tensor> A, B;   // factors carried between sweep steps
+    Leg bond;
+    double energy = 0;
+  };
+  Cursor cur_;
+
+ public:
+  // one two-site solve, split so that several runs can advance in lockstep
+  struct Solve {
+    int il = 0, ir = 0;
+    double p1 = 0, p2 = 0;
+    double t_step = 0, t_lanczos0 = 0;
+    StepOut o;                                 // fused legs, enlarged environments
+    std::unique_ptr<Tensor> psi;               // start vector, then the Ritz vector
+    std::unique_ptr<HeffPlan> plan;
+    std::vector<std::unique_ptr<Tensor>> phis; // lower states in this basis
+    int mlanc = 0;
+    bool fp32 = false;
+    int tag = kTagNone;
+    DevBuf pen;                                // device overlap of the penalty
+    ~Solve();
+  };
+  // adds the lower states' penalty -E_j <phi_j|x> phi_j to y (dmrg_su2.cpp:611-623)
+  void penalty(const Solve& S, const double* x, double* y);
+  LanczosResult solve(Solve& S);
+  SvdResult svd(Solve& S);
+  void release_plan(Solve& S);
+  // the stats counters a driver of several runs fills in for this one
+  DmrgStats& stats_mut() { return stats_; }
+
+ private:
+  std::vector<HeffTerm> heff_terms(int il, int ir, const Env& El, const Env& Er) const;
+  std::unique_ptr<Solve> prepare(int il, int ir, std::unique_ptr<Tensor> psi0, double p1,
+                                 double p2);
+  StepOut finish(Solve& S, const LanczosResult& lr, SvdResult&& sv);
   std::unique_ptr<Tensor> start_left(int nl, int nr, const Tensor& A, const Leg& b);
   std::unique_ptr<Tensor> start_right(int nl, int nr, const Tensor& B, const Leg& b);
   void ensure_left(int i);
@@ -173,6 +259,10 @@ class Dmrg {
   DmrgStats stats_;
   int last_bond_ = 0;
   int var1_ = -1, var2_ = -1;
+  int var32_[2] = {-1, -1};
+  Prec prec_ = Prec::F64;
+  double min_fp32_flops_ = kDefaultMinFp32Flops;
+  bool use_graph_ = true;
   int verbose_ = 0;
 };
 

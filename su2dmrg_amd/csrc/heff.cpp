@@ -15,12 +15,13 @@ namespace su2 {
 namespace {
 
 constexpr double kFacSkip = 1e-12;    // tensor_su2_dmrg_src.cpp:996
-// work units per launch (~8 and ~16 per CU on 256 CUs): stage 2 sums long K
-// ranges and splits them, and finer units balance its XCDs better (measured
-// on the 16x8 D=2000 centre bond: 4.35 ms at 2048 units, 3.99 ms at 4096,
-// flat up to 12288)
-constexpr int kTargetUnits1 = 2048;
-constexpr int kTargetUnits2 = 4096;
+// K splits of the two launches, per problem (GroupedGemm::finalize): stage 1
+// has short K (one sector dimension) and many tiles and is rarely split;
+// stage 2 sums long K ranges, and about 96 k-tiles per unit balance its XCDs
+// (measured on the 16x8 D=2000 centre bond, where the launch then has about
+// 4000 units: 4.35 ms at 2000 units, 3.99 ms at 4000, flat up to 12000)
+constexpr int kMinUnits1 = 128, kKtPerUnit1 = 128;
+constexpr int kMinUnits2 = 512, kKtPerUnit2 = 96;
 
 void check_op(const Tensor& op, const Leg& in, const Leg& out, int k, const char* what) {
   SU2_CHECK(op.nleg() == 3, kErrStructure, "%s operator must have 3 legs", what);
@@ -151,6 +152,17 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
         work += (long long)b.dim[0] * b.dim[0];
       }
     }
+  // H_R of each sector copied: the plan must not read the caller's tensor
+  // after it is built (the caller may free it)
+  std::map<int, long long> hr_off;
+  if (hr)
+    for (auto& b : psi.blocks()) {
+      const int j = b.twoj[1];
+      if (hr->find2(j, j) >= 0 && !hr_off.count(j)) {
+        hr_off[j] = work;
+        work += (long long)b.dim[1] * b.dim[1];
+      }
+    }
   d_in_ = c.dev_alloc_n<double>(std::max<long long>(psi_size_, 1));
   d_out_ = c.dev_alloc_n<double>(std::max<long long>(psi_size_, 1));
   if (work > 0) d_work_ = c.dev_alloc_n<double>(work);
@@ -197,58 +209,102 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
     transposes.push_back(TransposeDesc{d_work_ + kv.second, hl->data() + hb.offset, hb.dim[0]});
   }
   dev_transpose_batch(c, transposes);
+  for (auto& kv : hr_off) {
+    const Block& hb = hr->blocks()[hr->find2(kv.first, kv.first)];
+    dev_copy(c, d_work_ + kv.second, hr->data() + hb.offset, hb.size);
+  }
 
+  // The two launches as problems over three buffers (bundle workspace, input,
+  // output) with element offsets, instantiated for FP64 now and for FP32
+  // when a mixed-precision apply first asks for it.
   // stage 1: X(j->j') = f * Lb^T psi(j)   [T*D_a(j') x D_b(j)]
   for (auto& tr : trs) {
     const Block* bj = pb[tr.ja];
-    int p = stage1_.add_problem(d_work_ + tr.x_off, (long long)tr.nt * tr.Dap, tr.nt * tr.Dap,
-                                tr.Db);
-    stage1_.add_slice(p, d_work_ + tr.lb_off, tr.Da, 1, d_in_ + bj->offset, tr.Da, 0, tr.Da,
-                      tr.fac);
+    GemmSpec g{kBufWork, tr.x_off, (long long)tr.nt * tr.Dap, tr.nt * tr.Dap, tr.Db, {}};
+    g.slices.push_back(
+        SliceSpec{kBufWork, tr.lb_off, tr.Da, 1, kBufIn, bj->offset, tr.Da, tr.Da, tr.fac});
+    spec1_.push_back(g);
   }
   // stage 2: psi'(j') = sum_j X(j->j') Rb(j->j') + H_L^T psi + psi H_R
   for (auto& b : psi.blocks()) {
     const int jap = b.twoj[0];
     const int Dap = b.dim[0], Dbp = b.dim[1];
-    bool any = false;
-    int p = -1;
-    auto ensure = [&]() {
-      if (p < 0) p = stage2_.add_problem(d_out_ + b.offset, Dap, Dap, Dbp);
-      any = true;
-    };
+    GemmSpec g{kBufOut, b.offset, Dap, Dap, Dbp, {}};
     for (auto& tr : trs) {
       if (tr.jap != jap) continue;
-      ensure();
-      stage2_.add_slice(p, d_work_ + tr.x_off, Dap, 0, d_work_ + tr.rb_off,
-                        (long long)tr.nt * tr.Db, 0, tr.nt * tr.Db, 1.0);
+      g.slices.push_back(SliceSpec{kBufWork, tr.x_off, Dap, 0, kBufWork, tr.rb_off,
+                                   (long long)tr.nt * tr.Db, tr.nt * tr.Db, 1.0});
     }
-    if (hlt_off.count(jap)) {
-      ensure();
-      stage2_.add_slice(p, d_work_ + hlt_off.at(jap), Dap, 0, d_in_ + b.offset, Dap, 0, Dap, 1.0);
-    }
-    if (hr) {
-      int id = hr->find2(b.twoj[1], b.twoj[1]);
-      if (id >= 0) {
-        ensure();
-        stage2_.add_slice(p, d_in_ + b.offset, Dap, 0, hr->data() + hr->blocks()[id].offset, Dbp,
-                          0, Dbp, 1.0);
-      }
-    }
-    if (!any) zero_blocks_.push_back({b.offset, b.size});
+    if (hlt_off.count(jap))
+      g.slices.push_back(
+          SliceSpec{kBufWork, hlt_off.at(jap), Dap, 0, kBufIn, b.offset, Dap, Dap, 1.0});
+    if (hr_off.count(b.twoj[1]))
+      g.slices.push_back(
+          SliceSpec{kBufIn, b.offset, Dap, 0, kBufWork, hr_off.at(b.twoj[1]), Dbp, Dbp, 1.0});
+    if (g.slices.empty()) zero_blocks_.push_back({b.offset, b.size});
+    else spec2_.push_back(g);
   }
-  stage1_.finalize(c, kTargetUnits1);
-  stage2_.finalize(c, kTargetUnits2);
+  work_size_ = work;
+  void* bases[3] = {d_work_, d_in_, d_out_};
+  instantiate(stage1_, spec1_, bases, sizeof(double));
+  instantiate(stage2_, spec2_, bases, sizeof(double));
+  stage1_.finalize(c, kMinUnits1, kKtPerUnit1);
+  stage2_.finalize(c, kMinUnits2, kKtPerUnit2);
   SU2_HIP(hipStreamSynchronize(c.stream()));
+}
+
+void HeffPlan::instantiate(GroupedGemm& g, const std::vector<GemmSpec>& spec, void* const* bases,
+                           size_t es) {
+  auto ptr = [&](int buf, long long off) { return (void*)((char*)bases[buf] + off * es); };
+  for (auto& q : spec) {
+    const int p = g.add_problem(ptr(q.cbuf, q.coff), q.ldc, q.M, q.N);
+    for (auto& sl : q.slices)
+      g.add_slice(p, ptr(sl.abuf, sl.aoff), sl.lda, sl.ta, ptr(sl.bbuf, sl.boff), sl.ldb, 0, sl.K,
+                  sl.alpha);
+  }
+}
+
+void HeffPlan::ensure_fp32() {
+  if (stage1_32_) return;
+  ctx_.make_current();
+  // FP32 copies of the bundles and block Hamiltonians (the stage-1 output
+  // region is overwritten by every apply), input and output vectors
+  d_work32_ = (float*)pool_->alloc(sizeof(float) * std::max<long long>(work_size_, 1));
+  d_in32_ = (float*)pool_->alloc(sizeof(float) * std::max<long long>(psi_size_, 1));
+  d_out32_ = (float*)pool_->alloc(sizeof(float) * std::max<long long>(psi_size_, 1));
+  dev_convert_f64_f32(ctx_, d_work32_, d_work_, work_size_);
+  std::unique_ptr<GroupedGemm> s1(new GroupedGemm(Prec::F32)), s2(new GroupedGemm(Prec::F32));
+  void* bases[3] = {d_work32_, d_in32_, d_out32_};
+  instantiate(*s1, spec1_, bases, sizeof(float));
+  instantiate(*s2, spec2_, bases, sizeof(float));
+  s1->finalize(ctx_, kMinUnits1, kKtPerUnit1, var32_[0]);
+  s2->finalize(ctx_, kMinUnits2, kKtPerUnit2, var32_[1]);
+  stage1_32_ = std::move(s1);
+  stage2_32_ = std::move(s2);
+  SU2_HIP(hipStreamSynchronize(ctx_.stream()));
 }
 
 HeffPlan::~HeffPlan() {
   pool_->free(d_in_);
   pool_->free(d_out_);
   pool_->free(d_work_);
+  pool_->free(d_work32_);
+  pool_->free(d_in32_);
+  pool_->free(d_out32_);
 }
 
 void HeffPlan::apply_raw(const double* psi, double* out) {
   ctx_.make_current();
+  if (prec_ == Prec::F32) {
+    // FP32 MFMA with FP32 accumulation on FP32 copies; FP64 in and out
+    ensure_fp32();
+    dev_convert_f64_f32(ctx_, d_in32_, psi, psi_size_);
+    for (auto& z : zero_blocks_) dev_fill_f32(ctx_, d_out32_ + z.first, z.second, 0.0f);
+    stage1_32_->launch(ctx_.stream());
+    stage2_32_->launch(ctx_.stream());
+    dev_convert_f32_f64(ctx_, out, d_out32_, psi_size_);
+    return;
+  }
   dev_copy(ctx_, d_in_, psi, psi_size_);
   for (auto& z : zero_blocks_) dev_fill(ctx_, d_out_ + z.first, z.second, 0.0);
   stage1_.launch(ctx_.stream());
@@ -256,27 +312,48 @@ void HeffPlan::apply_raw(const double* psi, double* out) {
   dev_copy(ctx_, out, d_out_, psi_size_);
 }
 
+void HeffPlan::set_precision(Prec p) {
+  prec_ = p;
+  if (p == Prec::F32) ensure_fp32();
+}
+
+GroupedGemm& HeffPlan::stage(int i) {
+  if (prec_ == Prec::F32) {
+    ensure_fp32();
+    return i == 0 ? *stage1_32_ : *stage2_32_;
+  }
+  return i == 0 ? stage1_ : stage2_;
+}
+
 void HeffPlan::set_variants(int v1, int v2) {
   ctx_.make_current();
-  stage1_.set_variant(v1);
-  stage2_.set_variant(v2);
+  stage(0).set_variant(v1);
+  stage(1).set_variant(v2);
+  if (prec_ == Prec::F32) {
+    var32_[0] = v1;
+    var32_[1] = v2;
+  }
 }
 
 void HeffPlan::set_uniform(bool on) {
   ctx_.make_current();
-  stage1_.allow_uniform(on);
-  stage2_.allow_uniform(on);
-  stage1_.set_variant(stage1_.variant());
-  stage2_.set_variant(stage2_.variant());
+  for (int i = 0; i < 2; ++i) {
+    stage(i).allow_uniform(on);
+    stage(i).set_variant(stage(i).variant());
+  }
 }
 
 void HeffPlan::autotune(int reps, int* chosen, std::vector<float>* t1, std::vector<float>* t2) {
   ctx_.make_current();
   // stage 2 reads the stage-1 output; run once so its inputs are defined
-  stage1_.launch(ctx_.stream());
-  int a = stage1_.autotune(ctx_.stream(), reps, t1);
-  int b = stage2_.autotune(ctx_.stream(), reps, t2);
+  stage(0).launch(ctx_.stream());
+  int a = stage(0).autotune(ctx_.stream(), reps, t1);
+  int b = stage(1).autotune(ctx_.stream(), reps, t2);
   SU2_HIP(hipStreamSynchronize(ctx_.stream()));
+  if (prec_ == Prec::F32) {
+    var32_[0] = a;
+    var32_[1] = b;
+  }
   if (chosen) {
     chosen[0] = a;
     chosen[1] = b;

@@ -19,6 +19,7 @@
 // block-Hamiltonian products.  Both stages are single grouped launches.
 #pragma once
 
+#include <memory>
 #include <utility>
 #include <vector>
 
@@ -49,29 +50,66 @@ class HeffPlan {
   // Same, on raw device buffers of that layout (used by the Lanczos driver).
   void apply_raw(const double* psi, double* out);
 
+  // Arithmetic of apply: FP64 (default), or FP32 MFMA with FP32 accumulation
+  // on FP32 copies of the bundles and block Hamiltonians (made on first use),
+  // the input rounded to FP32 and the result widened to FP64.
+  void set_precision(Prec p);
+  Prec precision() const { return prec_; }
+
   double algorithmic_flops() const { return alg_flops_; }
   double executed_flops() const { return stage1_.flops() + stage2_.flops(); }
   long long psi_size() const { return psi_size_; }
+  // the two grouped launches of the current precision
+  GroupedGemm& stage(int i);
   const GroupedGemm& stage1() const { return stage1_; }
   const GroupedGemm& stage2() const { return stage2_; }
-  // Tile-variant choice of the two grouped launches: set explicitly, or time
-  // every compiled variant on this plan's own problems and keep the fastest.
+  // Tile-variant choice of the two grouped launches of the current
+  // precision: set explicitly, or time every compiled variant on this plan's
+  // own problems and keep the fastest.
   void set_variants(int v1, int v2);
   // Straight-line GEMM kernel for the single-layout launches (default) or
-  // the general kernel; both give the same bits.  Returns nothing; the
-  // layouts in use are stage1().layout(), stage2().layout().
+  // the general kernel; both give the same bits.
   void set_uniform(bool on);
   void autotune(int reps, int* chosen, std::vector<float>* t1, std::vector<float>* t2);
 
  private:
+  friend class HeffBatch;   // concatenates the launches of several plans
+  // a launch's problems over the plan's buffers, offsets in elements
+  enum { kBufWork = 0, kBufIn = 1, kBufOut = 2 };
+  struct SliceSpec {
+    int abuf;
+    long long aoff, lda;
+    int ta, bbuf;
+    long long boff, ldb;
+    int K;
+    double alpha;
+  };
+  struct GemmSpec {
+    int cbuf;
+    long long coff, ldc;
+    int M, N;
+    std::vector<SliceSpec> slices;
+  };
+  static void instantiate(GroupedGemm& g, const std::vector<GemmSpec>& spec, void* const* bases,
+                          size_t es);
+  void ensure_fp32();
+
   Context& ctx_;
   std::shared_ptr<DevicePool> pool_;   // frees the buffers even after ctx_ is gone
   long long psi_size_ = 0;
+  long long work_size_ = 0;
   double* d_in_ = nullptr;    // plan-owned copy of the input vector
   double* d_out_ = nullptr;   // plan-owned output vector
   double* d_work_ = nullptr;  // bundles + stage-1 intermediates
+  float* d_work32_ = nullptr; // FP32 mirrors (mixed precision)
+  float* d_in32_ = nullptr;
+  float* d_out32_ = nullptr;
   std::vector<std::pair<long long, long long>> zero_blocks_;  // (offset, size) without terms
+  std::vector<GemmSpec> spec1_, spec2_;
   GroupedGemm stage1_, stage2_;
+  std::unique_ptr<GroupedGemm> stage1_32_, stage2_32_;
+  int var32_[2] = {-1, -1};
+  Prec prec_ = Prec::F64;
   double alg_flops_ = 0;
 };
 

@@ -2,6 +2,7 @@
 #include "jacobi.hpp"
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -686,18 +687,23 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
         rounds[r].push_back(PairJob{b, P * CB, width(P), width(Q) ? Q * CB : 0, width(Q), 1});
       }
   }
-  // Row slices: a round with few pairs spreads each pair over up to
-  // MAX_SLICES workgroups (at least 64 rows each) so that it fills the chip.
+  // Row slices: a block with few pairs in a round spreads each pair over up
+  // to MAX_SLICES workgroups (at least 64 rows each) so that a round fills the
+  // chip.  The slice count of a pair follows from its own block (its pairs in
+  // the round, about a quarter of slice_target workgroups per block), never
+  // from the other blocks of the batch: the slices' partial Gram matrices
+  // are summed in slice order, so the slicing decides the bits.
   std::vector<int> round_off(rounds.size() + 1, 0), round_pmax(rounds.size(), 1);
   std::vector<PairJob> all;
   size_t gpart_max = 0;
   for (size_t r = 0; r < rounds.size(); ++r) {
     round_off[r] = (int)all.size();
     const int nj = (int)rounds[r].size();
-    // at most slice_target workgroups: one residency wave of the rotate kernel
-    const int pmax = nj == 0 ? 1 : std::min(MAX_SLICES, std::max(1, slice_target / nj));
+    std::map<int, int> per_block;
+    for (const PairJob& j : rounds[r]) per_block[j.blk]++;
     int used = 1;
     for (PairJob& j : rounds[r]) {
+      const int pmax = std::min(MAX_SLICES, std::max(1, slice_target / (4 * per_block[j.blk])));
       j.P = std::max(1, std::min(pmax, L[j.blk] / MIN_SLICE_ROWS));
       used = std::max(used, j.P);
     }

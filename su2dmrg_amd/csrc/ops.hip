@@ -38,7 +38,7 @@ void GemmBatch::run(Context& c, int variant) {
       for (int j = 0; j < N; ++j) dev_fill(c, z.first + (long long)j * ldc, M, 0.0);
   }
   if (n_ == 0) return;
-  g_.finalize(c, 1024, variant);
+  g_.finalize(c, 256, 64, variant);
   g_.launch(c.stream());
 }
 

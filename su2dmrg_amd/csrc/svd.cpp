@@ -12,96 +12,25 @@
 
 namespace su2 {
 
+
 namespace {
 
-// device buffer from the context's cache, returned when destroyed
-struct DevBuf {
-  Context* c = nullptr;
-  double* p = nullptr;
-  DevBuf() = default;
-  DevBuf(Context& ctx, long long n) : c(&ctx), p(ctx.dev_alloc_n<double>(n > 0 ? n : 0)) {}
-  ~DevBuf() {
-    if (c) c->dev_free(p);
-  }
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : c(o.c), p(o.p) { o.p = nullptr; }
-  DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) {
-      if (c) c->dev_free(p);
-      c = o.c;
-      p = o.p;
-      o.p = nullptr;
-    }
-    return *this;
-  }
+struct Sec {
+  int twoj, m, n, k, dc;
+  const double* A;
+  DevBuf V, U;
+  std::vector<double> s;   // k values, ascending, zeros for cut
 };
 
-}  // namespace
-
-SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int max_dcut) {
-  c.make_current();
-  SU2_CHECK(psi.nleg() == 2, kErrStructure, "svd: wavefunction must have two legs");
-  SU2_CHECK(max_dcut >= 1, kErrArgument, "svd: max_dcut=%d", max_dcut);
+// singular values, keep rule, global cut and output tensors of one
+// wavefunction whose sectors' triplets are in secs (Jacobi done)
+SvdResult finish_svd(Context& c, const Tensor& psi, std::vector<Sec>& secs,
+                     std::vector<std::vector<double>>& wv, double p1, double p2, int max_dcut) {
   const double tol = 1e-100;
-  struct Sec {
-    int twoj, m, n, k, dc;
-    const double* A;
-    DevBuf V, U;
-    std::vector<double> s;   // k values, ascending, zeros for cut
-  };
-  std::vector<Sec> secs;
-  for (auto& b : psi.blocks()) {
-    Sec s;
-    s.twoj = b.twoj[0];
-    s.m = b.dim[0];
-    s.n = b.dim[1];
-    s.k = std::min(s.m, s.n);
-    s.dc = 0;
-    s.A = psi.data() + b.offset;
-    secs.push_back(std::move(s));
-  }
-  // Singular triplets of every sector in one batched one-sided Jacobi SVD.
-  // The reference diagonalises the Gram matrix of the short side and
-  // re-diagonalises its small-eigenvalue subspace until accurate
-  // (tensor.cpp:1472-1529); rotating the sector's own columns reaches the same
-  // triplets without forming the Gram matrix.  "U" receives the long side
-  // (A V for m >= n, A^T U for m < n: the reference's u = A v), "V" the short
-  // side's eigenvectors, both in ascending order of the singular values.
-  std::vector<std::vector<double>> wv(secs.size());
-  int jacobi_sweeps = 0;
-  double t_jacobi = 0;
-  {
-    std::vector<JacobiBlock> jb;
-    std::vector<size_t> idx;
-    for (size_t i = 0; i < secs.size(); ++i) {
-      Sec& s = secs[i];
-      if (s.k == 0) continue;
-      const int L = std::max(s.m, s.n);
-      s.U = DevBuf(c, (long long)L * s.k);
-      s.V = DevBuf(c, (long long)s.k * s.k);
-      JacobiBlock b;
-      b.A = s.A;
-      b.m = s.m;
-      b.n = s.n;
-      b.W = s.U.p;
-      b.V = s.V.p;
-      jb.push_back(b);
-      idx.push_back(i);
-    }
-    const auto tj = std::chrono::steady_clock::now();
-    jacobi_sweeps = jacobi_svd_batched(c, jb, 1e-13, 40, c.svd_block_cols, c.svd_inner_sweeps);
-    t_jacobi = std::chrono::duration<double>(std::chrono::steady_clock::now() - tj).count();
-    for (size_t t = 0; t < jb.size(); ++t) {
-      std::vector<double>& w = wv[idx[t]];
-      w.resize(jb[t].s.size());
-      for (size_t q = 0; q < w.size(); ++q) w[q] = jb[t].s[q] * jb[t].s[q];
-    }
-  }
   // singular values and column scaling (tensor.cpp:1533-1568, 1620-1655)
   std::vector<double> wsort;
   std::vector<int> isort;
-  double* dfac = c.reduce_scratch(1);  // grown below
+  double* dfac = nullptr;
   for (size_t i = 0; i < secs.size(); ++i) {
     Sec& s = secs[i];
     if (s.k == 0) continue;
@@ -166,8 +95,6 @@ SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int
     }
   res.bond = make_leg(bt, bd, -1);
   res.discarded_weight = disc;
-  res.jacobi_sweeps = jacobi_sweeps;
-  res.t_jacobi = t_jacobi;
   for (size_t i = 0; i < secs.size(); ++i)
     for (int t = secs[i].k - keep[i]; t < secs[i].k; ++t) res.wout.push_back(secs[i].s[t]);
   for (size_t i = 0; i < secs.size(); ++i)
@@ -187,22 +114,15 @@ SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int
     // kept columns are the last d of the k (ascending singular values)
     const double* Ucols;
     const double* Vcols;
-    long long ldu, ldv;
     if (s.m >= s.n) {
       Ucols = s.U.p + (long long)(k - d) * s.m;
-      ldu = s.m;
       Vcols = s.V.p + (long long)(k - d) * k;
-      ldv = k;
     } else {
       Ucols = s.V.p + (long long)(k - d) * k;
-      ldu = k;
       Vcols = s.U.p + (long long)(k - d) * s.n;
-      ldv = s.n;
     }
     const int bu = res.U->find2(s.twoj, s.twoj), bv = res.V->find2(s.twoj, s.twoj);
     SU2_CHECK(bu >= 0 && bv >= 0, kErrStructure, "svd: missing output block");
-    (void)ldu;
-    (void)ldv;
     dev_copy(c, res.U->data() + res.U->blocks()[bu].offset, Ucols, (long long)s.m * d);
     dev_copy(c, res.V->data() + res.V->blocks()[bv].offset, Vcols, (long long)s.n * d);
     std::vector<double> kv(s.s.begin() + (k - d), s.s.end());
@@ -210,6 +130,84 @@ SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int
   }
   SU2_HIP(hipStreamSynchronize(c.stream()));
   return res;
+}
+
+}  // namespace
+
+std::vector<SvdResult> truncated_svd_batch(Context& c, const std::vector<SvdJob>& jobs) {
+  c.make_current();
+  std::vector<std::vector<Sec>> secs(jobs.size());
+  for (size_t q = 0; q < jobs.size(); ++q) {
+    const SvdJob& J = jobs[q];
+    SU2_CHECK(J.psi && J.psi->nleg() == 2, kErrStructure, "svd: wavefunction must have two legs");
+    SU2_CHECK(J.max_dcut >= 1, kErrArgument, "svd: max_dcut=%d", J.max_dcut);
+    for (auto& b : J.psi->blocks()) {
+      Sec s;
+      s.twoj = b.twoj[0];
+      s.m = b.dim[0];
+      s.n = b.dim[1];
+      s.k = std::min(s.m, s.n);
+      s.dc = 0;
+      s.A = J.psi->data() + b.offset;
+      secs[q].push_back(std::move(s));
+    }
+  }
+  // Singular triplets of every sector of every wavefunction in one batched
+  // one-sided Jacobi SVD.  The reference diagonalises the Gram matrix of the
+  // short side and re-diagonalises its small-eigenvalue subspace until
+  // accurate (tensor.cpp:1472-1529); rotating the sector's own columns
+  // reaches the same triplets without forming the Gram matrix.  "U" receives
+  // the long side (A V for m >= n, A^T U for m < n: the reference's u = A v),
+  // "V" the short side's eigenvectors, both in ascending order of the
+  // singular values.  Each block's rotations depend on that block alone, so
+  // the batch gives every wavefunction the bits it has by itself.
+  std::vector<std::vector<std::vector<double>>> wv(jobs.size());
+  int jacobi_sweeps = 0;
+  double t_jacobi = 0;
+  {
+    std::vector<JacobiBlock> jb;
+    std::vector<std::pair<size_t, size_t>> idx;
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      wv[q].resize(secs[q].size());
+      for (size_t i = 0; i < secs[q].size(); ++i) {
+        Sec& s = secs[q][i];
+        if (s.k == 0) continue;
+        const int L = std::max(s.m, s.n);
+        s.U = DevBuf(c, (long long)L * s.k);
+        s.V = DevBuf(c, (long long)s.k * s.k);
+        JacobiBlock b;
+        b.A = s.A;
+        b.m = s.m;
+        b.n = s.n;
+        b.W = s.U.p;
+        b.V = s.V.p;
+        jb.push_back(b);
+        idx.push_back({q, i});
+      }
+    }
+    const auto tj = std::chrono::steady_clock::now();
+    jacobi_sweeps = jacobi_svd_batched(c, jb, 1e-13, 40, c.svd_block_cols, c.svd_inner_sweeps);
+    t_jacobi = std::chrono::duration<double>(std::chrono::steady_clock::now() - tj).count();
+    for (size_t t = 0; t < jb.size(); ++t) {
+      std::vector<double>& w = wv[idx[t].first][idx[t].second];
+      w.resize(jb[t].s.size());
+      for (size_t q = 0; q < w.size(); ++q) w[q] = jb[t].s[q] * jb[t].s[q];
+    }
+  }
+  std::vector<SvdResult> out;
+  for (size_t q = 0; q < jobs.size(); ++q) {
+    out.push_back(finish_svd(c, *jobs[q].psi, secs[q], wv[q], jobs[q].p1, jobs[q].p2,
+                             jobs[q].max_dcut));
+    out.back().jacobi_sweeps = jacobi_sweeps;
+    out.back().t_jacobi = t_jacobi / jobs.size();
+  }
+  return out;
+}
+
+SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int max_dcut) {
+  std::vector<SvdJob> jobs{SvdJob{&psi, p1, p2, max_dcut}};
+  std::vector<SvdResult> r = truncated_svd_batch(c, jobs);
+  return std::move(r[0]);
 }
 
 }  // namespace su2

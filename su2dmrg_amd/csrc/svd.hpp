@@ -35,4 +35,14 @@ struct SvdResult {
 
 SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int max_dcut);
 
+// Several wavefunctions at once: every sector of every one in one batched
+// Jacobi SVD, then each one's keep rule and cut.  Each result equals the
+// single call's (its blocks' rotations depend on those blocks alone).
+struct SvdJob {
+  const Tensor* psi;
+  double p1, p2;
+  int max_dcut;
+};
+std::vector<SvdResult> truncated_svd_batch(Context& c, const std::vector<SvdJob>& jobs);
+
 }  // namespace su2

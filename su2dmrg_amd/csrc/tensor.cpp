@@ -1,5 +1,7 @@
 #include "tensor.hpp"
 
+#include <algorithm>
+
 #include "common.hpp"
 #include "context.hpp"
 
@@ -58,6 +60,8 @@ void* DevicePool::alloc(size_t bytes) {
     if (it == free_.end() || it->first > 2 * sz) return nullptr;
     void* p = it->second;
     live_[p] = it->first;
+    live_bytes_ += it->first;
+    peak_bytes_ = std::max(peak_bytes_, live_bytes_);
     free_.erase(it);
     return p;
   };
@@ -76,19 +80,32 @@ void* DevicePool::alloc(size_t bytes) {
     SU2_HIP(hipMalloc(&p, sz));
   }
   live_[p] = sz;
+  live_bytes_ += sz;
+  peak_bytes_ = std::max(peak_bytes_, live_bytes_);
   return p;
+}
+
+size_t DevicePool::cached_bytes() const {
+  size_t b = 0;
+  for (auto& x : free_) b += x.first;
+  for (auto& x : held_) b += x.first;
+  return b;
 }
 
 void DevicePool::free(void* p) {
   if (!p) return;
   auto it = live_.find(p);
   if (detached_ || it == live_.end()) {   // context gone (or not a pool block)
-    if (it != live_.end()) live_.erase(it);
+    if (it != live_.end()) {
+      live_bytes_ -= it->second;
+      live_.erase(it);
+    }
     (void)hipSetDevice(device_);
     (void)hipFree(p);
     return;
   }
   held_.push_back({it->second, p});
+  live_bytes_ -= it->second;
   live_.erase(it);
 }
 
@@ -97,8 +114,13 @@ void DevicePool::set_stream(hipStream_t s) {
   stream_ = s;
 }
 
-void DevicePool::detach() {
-  recycle();
+void DevicePool::detach() noexcept {
+  // runs from ~Context: a sticky device error must not throw here, so the
+  // wait's status is dropped (su2_ctx_destroy waits and reports it first)
+  (void)hipSetDevice(device_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto& x : held_) free_.insert({x.first, x.second});
+  held_.clear();
   release_free();
   stream_ = nullptr;
   detached_ = true;

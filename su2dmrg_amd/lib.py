@@ -38,6 +38,7 @@ _SIG = {
     "su2_last_error": (C.c_char_p, [_p]),
     "su2_ctx_synchronize": (_i, [_p]),
     "su2_ctx_set_svd_tuning": (_i, [_p, _i, _i]),
+    "su2_ctx_memory": (_i, [_p, _llp, _llp, _llp, _i]),
     "su2_tensor_create": (_i, [_p, _i, _ip, _ip, _ip, _ip, _i, _ip, C.POINTER(_p)]),
     "su2_tensor_destroy": (_i, [_p, _p]),
     "su2_tensor_size": (_i, [_p, _llp, _ip]),
@@ -58,6 +59,7 @@ _SIG = {
     "su2_heff_autotune": (_i, [_p, _p, _i, _ip, C.POINTER(C.c_float), _ip]),
     "su2_heff_set_variants": (_i, [_p, _p, _i, _i]),
     "su2_heff_set_kernel_path": (_i, [_p, _p, _i, _p]),
+    "su2_heff_set_precision": (_i, [_p, _p, _i]),
     "su2_svd_truncate": (_i, [_p, _p, _d, _d, _i, C.POINTER(_p), C.POINTER(_p), _dp, _i, _ip,
                               _dp]),
     "su2_recoupling": (_i, [_i, _ip, _dp]),
@@ -74,6 +76,23 @@ _SIG = {
     "su2_dmrg_load": (_i, [_p, _p, C.c_char_p, _i, _i, _i, _dp]),
     "su2_dmrg_add_lower_state": (_i, [_p, _p, C.c_char_p, _i, _i, _i]),
     "su2_dmrg_stats": (_i, [_p, _dp, _i]),
+    "su2_dmrg_set_matvec_precision": (_i, [_p, _p, _i, _d]),
+    "su2_dmrg_set_graph": (_i, [_p, _p, _i]),
+    "su2_batch_create": (_i, [_p, C.POINTER(_p)]),
+    "su2_batch_destroy": (_i, [_p, _p]),
+    "su2_batch_add": (_i, [_p, _p, _p, _i, _ip, _ip, _d, _d, _ip]),
+    "su2_batch_tick": (_i, [_p, _p, _ip]),
+    "su2_batch_run": (_i, [_p, _p]),
+    "su2_batch_set_graph": (_i, [_p, _p, _i]),
+    "su2_batch_energies": (_i, [_p, _i, _ip, _dp, _i, _ip]),
+    "su2_batch_stats": (_i, [_p, _dp, _i]),
+    "su2_server_create": (_i, [_p, _d, _i, C.c_char_p, C.POINTER(_p)]),
+    "su2_server_destroy": (_i, [_p, _p]),
+    "su2_server_submit": (_i, [_p, _p, _p, _d, _ip]),
+    "su2_server_run": (_i, [_p, _p, _i]),
+    "su2_server_result": (_i, [_p, _i, _dp, _dp, _dp]),
+    "su2_server_stats": (_i, [_p, _dp, _i]),
+    "su2_job_footprint": (_i, [_p, _dp]),
 }
 
 for _name, (_res, _args) in _SIG.items():
@@ -89,10 +108,35 @@ def declared_symbols():
     return sorted(set(re.findall(r"\b(su2_[a-z0-9_]+)\s*\(", text)))
 
 
+class Job(C.Structure):
+    """su2_job: one served DMRG job (the reference's run flags)."""
+    _fields_ = [("lx", C.c_int), ("ly", C.c_int), ("jcoup", C.c_double), ("qcoup", C.c_double),
+                ("bdry", C.c_int), ("totspin", C.c_int), ("max_dcut", C.c_int),
+                ("exci", C.c_int), ("nsweep", C.c_int), ("stop", C.c_double)]
+
+    def __init__(self, lx, ly, jcoup=2.0, qcoup=0.0, bdry=2, totspin=0, max_dcut=100, exci=0,
+                 nsweep=2, stop=-1.0):
+        super().__init__(lx, ly, float(jcoup), float(qcoup), bdry, totspin, max_dcut, exci,
+                         nsweep, float(stop))
+
+    def footprint(self):
+        b = C.c_double()
+        rc = lib.su2_job_footprint(C.byref(self), C.byref(b))
+        if rc:
+            raise Su2Error(rc, lib.su2_last_error(None).decode())
+        return b.value
+
+
 class Su2Error(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"su2 status {code}: {msg}")
         self.code = code
+
+
+def _check_global(rc):
+    """Status of a call that takes no context (its message is the thread's last)."""
+    if rc:
+        raise Su2Error(rc, lib.su2_last_error(None).decode())
 
 
 def _ia(x):
@@ -121,6 +165,13 @@ class Context:
         """Column block width (16 or 32) and inner sweeps per pair visit (1..3)
         of the one-sided Jacobi SVD."""
         self.check(lib.su2_ctx_set_svd_tuning(self.h, int(block_cols), int(inner_sweeps)))
+
+    def memory(self, reset_peak=False):
+        """(live, peak, cached) device bytes of the context's cache."""
+        a, b, c = _ll(), _ll(), _ll()
+        self.check(lib.su2_ctx_memory(self.h, C.byref(a), C.byref(b), C.byref(c),
+                                      int(bool(reset_peak))))
+        return a.value, b.value, c.value
 
     def close(self):
         if self.h:
@@ -301,7 +352,7 @@ class HeffPlan:
 
     def flops(self):
         a, e = _d(), _d()
-        self.ctx.check(lib.su2_heff_flops(self.h, C.byref(a), C.byref(e)))
+        _check_global(lib.su2_heff_flops(self.h, C.byref(a), C.byref(e)))
         return a.value, e.value
 
     def time(self, psi, out, reps):
@@ -314,7 +365,7 @@ class HeffPlan:
         """Time every tile variant for both launches; returns (chosen, times1, times2)."""
         ch = (C.c_int * 2)()
         nv = C.c_int()
-        times = (C.c_float * 64)()
+        times = (C.c_float * 128)()
         self.ctx.check(lib.su2_heff_autotune(self.ctx.h, self.h, int(reps), ch, times,
                                              C.byref(nv)))
         n = nv.value
@@ -322,6 +373,10 @@ class HeffPlan:
 
     def set_variants(self, v1, v2):
         self.ctx.check(lib.su2_heff_set_variants(self.ctx.h, self.h, int(v1), int(v2)))
+
+    def set_precision(self, bits):
+        """64: FP64 matvec (default); 32: FP32 MFMA with FP32 accumulation."""
+        self.ctx.check(lib.su2_heff_set_precision(self.ctx.h, self.h, int(bits)))
 
     def set_kernel_path(self, uniform):
         """Straight-line kernel for single-layout launches (True) or the
@@ -456,33 +511,62 @@ class Dmrg:
                                                     int(file_dcut), ky, exci))
 
     def set_verbose(self, level=1):
-        self.ctx.check(lib.su2_dmrg_set_verbose(self.h, int(level)))
+        _check_global(lib.su2_dmrg_set_verbose(self.h, int(level)))
 
     def set_heff_variants(self, v1, v2):
         self.ctx.check(lib.su2_dmrg_set_heff_variants(self.ctx.h, self.h, int(v1), int(v2)))
 
     def energies(self):
         n = _i()
-        self.ctx.check(lib.su2_dmrg_energies(self.h, None, 0, C.byref(n)))
+        _check_global(lib.su2_dmrg_energies(self.h, None, 0, C.byref(n)))
         buf = (C.c_double * max(1, n.value))()
-        self.ctx.check(lib.su2_dmrg_energies(self.h, buf, n.value, C.byref(n)))
+        _check_global(lib.su2_dmrg_energies(self.h, buf, n.value, C.byref(n)))
         return list(buf[:n.value])
 
     def history(self):
         """Per step: (kept multiplets, discarded weight, matvecs)."""
         n = _i()
-        self.ctx.check(lib.su2_dmrg_history(self.h, None, None, None, 0, C.byref(n)))
+        _check_global(lib.su2_dmrg_history(self.h, None, None, None, 0, C.byref(n)))
         m = max(1, n.value)
         kept, disc, mv = (_i * m)(), (C.c_double * m)(), (_i * m)()
-        self.ctx.check(lib.su2_dmrg_history(self.h, kept, disc, mv, n.value, C.byref(n)))
+        _check_global(lib.su2_dmrg_history(self.h, kept, disc, mv, n.value, C.byref(n)))
         return [(kept[i], disc[i], mv[i]) for i in range(n.value)]
 
+    def set_graph(self, on=True):
+        self.ctx.check(lib.su2_dmrg_set_graph(self.ctx.h, self.h, int(bool(on))))
+
+    def set_matvec_precision(self, bits, min_flops=-1.0):
+        """64: FP64 matvecs; 32: FP32 MFMA Krylov matvecs with FP64 checks, for
+        solves whose matvec has at least `min_flops` FLOPs (< 0: default)."""
+        self.ctx.check(lib.su2_dmrg_set_matvec_precision(self.ctx.h, self.h, int(bits),
+                                                         float(min_flops)))
+
     def stats(self):
-        out = (C.c_double * 12)()
-        self.ctx.check(lib.su2_dmrg_stats(self.h, out, 12))
+        out = (C.c_double * 16)()
+        _check_global(lib.su2_dmrg_stats(self.h, out, 16))
         keys = ["t_env", "t_lanczos", "t_svd", "t_project", "t_start", "t_matvec",
-                "heff_flops", "matvecs", "steps", "bond_dim", "t_jacobi", "jacobi_sweeps"]
+                "heff_flops", "matvecs", "steps", "bond_dim", "t_jacobi", "jacobi_sweeps",
+                "matvecs32", "t_matvec32", "heff_flops32", "max_fp64_correction"]
         return dict(zip(keys, list(out)))
+
+    def run_schedule(self, schedule, stop=None, min_flops=-1.0):
+        """Warm-up, then one sweep per (max_dcut, bits) entry of `schedule`
+        (the warm-up uses the first entry); returns the energy after each
+        sweep.  With `stop`, ends early once |dE| < stop (main_omp.cpp:197)."""
+        d0, b0 = schedule[0]
+        self.set_max_dcut(d0)
+        self.set_matvec_precision(b0, min_flops)
+        prev = self.warmup()
+        out = []
+        for d, bits in schedule:
+            self.set_max_dcut(d)
+            self.set_matvec_precision(bits, min_flops)
+            e = self.sweep()
+            out.append(e)
+            if stop is not None and abs(e - prev) < stop:
+                break
+            prev = e
+        return out
 
     def run(self, nsweep=5, stop=1e-6):
         """Warm-up, then sweeps until |dE| < stop (main_omp.cpp:184-202)."""
@@ -499,6 +583,111 @@ class Dmrg:
     def close(self):
         if self.h:
             lib.su2_dmrg_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """su2_batch: several Dmrg runs of one context advanced in lockstep."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = _p()
+        ctx.check(lib.su2_batch_create(ctx.h, C.byref(h)))
+        self.h = h
+        self._runs = []
+
+    def add(self, dmrg, schedule, stop=-1.0, min_flops=-1.0):
+        """schedule: list of (max_dcut, bits) sweeps; the warm-up uses entry 0."""
+        n = len(schedule)
+        d, _ = _ia([x[0] for x in schedule] or [0])
+        b, _ = _ia([x[1] for x in schedule] or [64])
+        rid = _i()
+        self.ctx.check(lib.su2_batch_add(self.ctx.h, self.h, dmrg.h, n, d.ctypes.data_as(_ip),
+                                         b.ctypes.data_as(_ip), float(min_flops), float(stop),
+                                         C.byref(rid)))
+        self._runs.append(dmrg)
+        return rid.value
+
+    def tick(self):
+        n = _i()
+        self.ctx.check(lib.su2_batch_tick(self.ctx.h, self.h, C.byref(n)))
+        return n.value
+
+    def run(self):
+        self.ctx.check(lib.su2_batch_run(self.ctx.h, self.h))
+
+    def set_graph(self, on=True):
+        self.ctx.check(lib.su2_batch_set_graph(self.ctx.h, self.h, int(bool(on))))
+
+    def energies(self, rid):
+        """(finished, energy after each finished phase, warm-up first)."""
+        f, n = _i(), _i()
+        buf = (C.c_double * 256)()
+        _check_global(lib.su2_batch_energies(self.h, int(rid), C.byref(f), buf, 256, C.byref(n)))
+        return bool(f.value), list(buf[:min(n.value, 256)])
+
+    def stats(self):
+        out = (C.c_double * 8)()
+        _check_global(lib.su2_batch_stats(self.h, out, 8))
+        keys = ["ticks", "graphs_built", "graph_replays", "t_prepare", "t_lanczos", "t_matvec",
+                "t_svd", "t_complete"]
+        return dict(zip(keys, list(out)))
+
+    def close(self):
+        if self.h:
+            lib.su2_batch_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """su2_server: request loop with continuous batching on one device."""
+
+    def __init__(self, ctx, budget_bytes, max_slots, workdir):
+        self.ctx = ctx
+        h = _p()
+        ctx.check(lib.su2_server_create(ctx.h, float(budget_bytes), int(max_slots),
+                                        os.fsencode(workdir), C.byref(h)))
+        self.h = h
+        self.n = 0
+
+    def submit(self, job, arrival_s=0.0):
+        jid = _i()
+        self.ctx.check(lib.su2_server_submit(self.ctx.h, self.h, C.byref(job), float(arrival_s),
+                                             C.byref(jid)))
+        self.n += 1
+        return jid.value
+
+    def run(self, verbose=0):
+        self.ctx.check(lib.su2_server_run(self.ctx.h, self.h, int(verbose)))
+
+    def result(self, jid):
+        """dict(energy, t_arrive, t_admit, t_finish, latency, footprint)."""
+        e, fp = _d(), _d()
+        t = (C.c_double * 3)()
+        _check_global(lib.su2_server_result(self.h, int(jid), C.byref(e), t, C.byref(fp)))
+        return {"energy": e.value, "t_arrive": t[0], "t_admit": t[1], "t_finish": t[2],
+                "latency": t[2] - t[0], "footprint": fp.value}
+
+    def stats(self):
+        out = (C.c_double * 3)()
+        _check_global(lib.su2_server_stats(self.h, out, 3))
+        return {"wall": out[0], "ticks": out[1], "max_concurrent": out[2]}
+
+    def close(self):
+        if self.h:
+            lib.su2_server_destroy(self.ctx.h, self.h)
             self.h = None
 
     def __del__(self):

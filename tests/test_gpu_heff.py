@@ -206,3 +206,106 @@ def test_vector_algebra(ctx):
     tw.copy_from(tu)
     tw.scal(3.0)
     assert abs(tw.dot(tw) - 9.0) < 1e-12
+
+
+@pytest.fixture(scope="module")
+def headline():
+    """16x8 D=2000 centre-bond workload (the benchmarked matvec) and its oracle result."""
+    work = synthetic.cyl16x8_d2000(seed=0)
+    ref = oheff.heff_apply(work["psi"], work["hl"], work["hr"], work["terms"])
+    return work, ref
+
+
+def test_heff_cyl16x8_d2000_default_variants_match_oracle(ctx, headline):
+    # reaches what only the headline size does: stage 2 split into ~4000 work
+    # units, ~8000 workgroups through the XCD remap, 32-bit byte offsets on
+    # 10896 x 1362 operands
+    _say("heff cyl16x8_d2000 seed=0, default tile variants")
+    work, ref = headline
+    dev, got = _run(ctx, work)
+    assert _relerr(got, ref) < RTOL
+    alg, _ = dev["plan"].flops()
+    assert alg == synthetic.flops(work)
+
+
+def test_heff_cyl16x8_d2000_autotuned_matches_oracle(ctx, headline):
+    from su2dmrg_amd import lib as L
+    _say("heff cyl16x8_d2000 seed=0, autotuned tile variants")
+    work, ref = headline
+    dev = L.upload_workload(ctx, work)
+    chosen, _, _ = dev["plan"].autotune(reps=1)
+    _say(f"  autotuned variants {chosen}")
+    dev["plan"].apply(dev["psi"], dev["out"])
+    got = L.unpack_wavefunction(dev["out"], dev["out"].download())
+    assert _relerr(got, ref) < RTOL
+
+
+def test_heff_plan_outlives_block_hamiltonians(ctx):
+    # the plan keeps its own copies of H_L and H_R: freeing the caller's
+    # tensors (and reusing their memory) must not change later applies
+    from su2dmrg_amd import lib as L
+    _say("heff plan after hl/hr are destroyed")
+    work = synthetic.small(seed=5, twoj=(1, 3, 5), dims=(60, 90, 20), nterm=2)
+    dev = L.upload_workload(ctx, work)
+    dev["hl"].close()
+    dev["hr"].close()
+    junk = [L.scalar_operator(ctx, work["twoj"], work["dims_a"],
+                              {j: np.full((d, d), 7.0) for j, d in zip(work["twoj"],
+                                                                      work["dims_a"])})
+            for _ in range(2)]
+    dev["plan"].apply(dev["psi"], dev["out"])
+    got = L.unpack_wavefunction(dev["out"], dev["out"].download())
+    ref = oheff.heff_apply(work["psi"], work["hl"], work["hr"], work["terms"])
+    assert _relerr(got, ref) < RTOL
+    for j in junk:
+        j.close()
+
+
+# FP32 MFMA matvec (FP32 accumulation over K up to ~25000 terms): a
+# relative 2-norm error of a few 1e-6 is expected (rounding of the operands
+# to FP32, 2.4e-8 relative rms, plus accumulation error growing with
+# sqrt(K / 4)); the bound is 5e-5.  The lower bound shows the FP32 path ran.
+RTOL_FP32 = 5e-5
+
+
+@pytest.mark.parametrize("seed,twoj,dims,nterm", [
+    (0, (0, 1, 2, 3, 4), (3, 4, 5, 2, 1), 3),
+    (2, (0, 2, 4, 6), (130, 257, 129, 7), 4),
+])
+def test_heff_fp32_small_matches_oracle(ctx, seed, twoj, dims, nterm):
+    from su2dmrg_amd import lib as L
+    _say(f"heff fp32 seed={seed} twoj={twoj} dims={dims}")
+    work = synthetic.small(seed=seed, twoj=twoj, dims=dims, nterm=nterm)
+    dev = L.upload_workload(ctx, work)
+    dev["plan"].set_precision(32)
+    dev["plan"].apply(dev["psi"], dev["out"])
+    got = L.unpack_wavefunction(dev["out"], dev["out"].download())
+    ref = oheff.heff_apply(work["psi"], work["hl"], work["hr"], work["terms"])
+    err = _relerr(got, ref)
+    _say(f"  fp32 rel err {err:.3e}")
+    assert 1e-10 < err < RTOL_FP32
+    # back to FP64 on the same plan
+    dev["plan"].set_precision(64)
+    dev["plan"].apply(dev["psi"], dev["out"])
+    got = L.unpack_wavefunction(dev["out"], dev["out"].download())
+    assert _relerr(got, ref) < RTOL
+
+
+def test_heff_fp32_cyl16x8_d2000_every_variant(ctx, headline):
+    from su2dmrg_amd import lib as L
+    _say("heff fp32 cyl16x8_d2000 seed=0, every tile variant")
+    work, ref = headline
+    dev = L.upload_workload(ctx, work)
+    plan = dev["plan"]
+    plan.set_precision(32)
+    chosen, t1, t2 = plan.autotune(reps=1)
+    _say(f"  fp32 autotune {chosen} stage1 {['%.2f' % t for t in t1]} "
+         f"stage2 {['%.2f' % t for t in t2]}")
+    nv = len(t1)
+    for v in range(nv):
+        plan.set_variants(v, v)
+        plan.apply(dev["psi"], dev["out"])
+        got = L.unpack_wavefunction(dev["out"], dev["out"].download())
+        err = _relerr(got, ref)
+        _say(f"  variant {v}: fp32 rel err {err:.3e}")
+        assert 1e-10 < err < RTOL_FP32
