@@ -103,6 +103,19 @@ int su2_tensor_axpy(su2_ctx* ctx, su2_tensor* y, double a, const su2_tensor* x);
 int su2_tensor_scal(su2_ctx* ctx, su2_tensor* x, double a);
 int su2_tensor_dot(su2_ctx* ctx, const su2_tensor* x, const su2_tensor* y, double* out);
 int su2_tensor_normalize(su2_ctx* ctx, su2_tensor* x, double* norm);
+/* Allocation-free su2_tensor_dot.  No counterpart in the reference: it is for
+ * callers that cannot allocate device or pinned memory (code under HIP graph
+ * capture, tight serving loops).  Same kernels and summation order as
+ * su2_tensor_dot, hence the same bits; the partial sums go to a fixed
+ * per-GPU array of su2_noalloc_scratch_bytes() bytes (4104) that the library
+ * holds until the process ends, so the call makes no hipMalloc, hipFree or
+ * pinned allocation.  The partial count is fixed, so tensors of every size
+ * fit it.  Calls on one GPU share that array (it keeps the last call's
+ * contents): calls that overlap on one GPU, from several threads or
+ * contexts, race.  The context must have been created (su2_ctx_create
+ * resolves the array's address for its device). */
+int su2_tensor_dot_noalloc(su2_ctx* ctx, const su2_tensor* x, const su2_tensor* y, double* out);
+int su2_noalloc_scratch_bytes(long long* bytes);
 
 /* H_eff plan (dmrg_su2::hamiltonian_vector_multiplication_idmrg,
  * dmrg_su2.cpp:543-664, and tensor_su2::hamiltonian_vector_multiplication,

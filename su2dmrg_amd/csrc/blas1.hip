@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "common.hpp"
 #include "context.hpp"
@@ -284,8 +285,59 @@ unsigned grid_for(long long n) {
   return (unsigned)b;
 }
 
+// Scratch of the allocation-free dot: the kDotBlocks partial sums and the
+// result.  One copy per GPU, held until the process ends; host code reaches
+// it only through the address hipGetSymbolAddress returns for the current
+// device (noalloc_scratch), never through this name.
+constexpr int kNoAllocScratchDoubles = kDotBlocks + 1;
+__device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;   // by device ordinal; null until resolved
+
 }  // namespace
 
+void noalloc_scratch_init(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  if (g_scratch_ptr[device]) return;
+  int cur = -1;
+  SU2_HIP(hipGetDevice(&cur));
+  SU2_CHECK(cur == device, kErrArgument, "noalloc scratch: device %d is not current (%d)", device,
+            cur);
+  double* ptr = nullptr;
+  SU2_HIP(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+  SU2_CHECK(ptr != nullptr, kErrDevice, "noalloc scratch: no address on device %d", device);
+  g_scratch_ptr[device] = ptr;
+}
+
+double dev_dot_noalloc(Context& c, const double* x, const double* y, long long n) {
+  double* part = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (c.device() < (int)g_scratch_ptr.size()) part = g_scratch_ptr[c.device()];
+  }
+  SU2_CHECK(part != nullptr, kErrArgument,
+            "noalloc dot: scratch of device %d not resolved (context not initialised)", c.device());
+  // the partial count is fixed, so every length fits the scratch
+  static_assert(kDotBlocks + 1 <= kNoAllocScratchDoubles, "dot scratch too small");
+  double res = 0.0;
+  if (n > 0) {
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(kDotBlocks), dim3(kThreads), 0, c.stream(), x, y,
+                       n, part);
+    SU2_HIP(hipGetLastError());
+    hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kThreads), 0, c.stream(), part,
+                       part + kDotBlocks);
+    SU2_HIP(hipGetLastError());
+    // pageable destination: the copy completes before the call returns
+    SU2_HIP(hipMemcpyAsync(&res, part + kDotBlocks, sizeof(double), hipMemcpyDeviceToHost,
+                           c.stream()));
+  }
+  SU2_HIP(hipStreamSynchronize(c.stream()));
+  return res;
+}
+
+long long noalloc_scratch_bytes() { return (long long)sizeof(double) * kNoAllocScratchDoubles; }
 
 void dev_fill(Context& c, double* x, long long n, double v) {
   if (n <= 0) return;

@@ -40,6 +40,15 @@ void dev_axpy(Context& c, double* y, const double* x, long long n, double a);   
 void dev_axpby(Context& c, double* y, const double* x, long long n, double a, double b);  // y = a x + b y
 // sum_i x_i y_i (synchronises the context stream and returns on the host)
 double dev_dot(Context& c, const double* x, const double* y, long long n);
+// Allocation-free variant of dev_dot: the same kernels and summation order,
+// with the partials in a fixed per-device array (noalloc_scratch_bytes bytes)
+// instead of the context's scratch; no device or pinned allocation happens.
+// noalloc_scratch_init(device) must have run once with that device current
+// (Context's constructor does it).  Calls on one device share the array, so
+// overlapping calls from several threads or streams race.
+void noalloc_scratch_init(int device);
+double dev_dot_noalloc(Context& c, const double* x, const double* y, long long n);
+long long noalloc_scratch_bytes();
 // the same sum, left on the device in *res (stream-ordered, no wait)
 void dev_dot_to(Context& c, const double* x, const double* y, long long n, double* res);
 // y += (sign * a[0]) x, the coefficient read on the device (no wait)

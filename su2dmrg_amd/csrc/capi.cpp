@@ -276,6 +276,25 @@ int su2_tensor_dot(su2_ctx* ctx, const su2_tensor* x, const su2_tensor* y, doubl
   });
 }
 
+int su2_tensor_dot_noalloc(su2_ctx* ctx, const su2_tensor* x, const su2_tensor* y, double* out) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(x, "x");
+    need(y, "y");
+    need(out, "out");
+    same_layout(x->t, y->t, "dot");
+    ctx->c.make_current();
+    *out = su2::dev_dot_noalloc(ctx->c, x->t.data(), y->t.data(), x->t.size());
+  });
+}
+
+int su2_noalloc_scratch_bytes(long long* bytes) {
+  return guard(nullptr, [&] {
+    need(bytes, "bytes");
+    *bytes = su2::noalloc_scratch_bytes();
+  });
+}
+
 int su2_tensor_normalize(su2_ctx* ctx, su2_tensor* x, double* norm) {
   return guard(ctx, [&] {
     need(ctx, "ctx");

@@ -2,6 +2,7 @@
 
 #include <algorithm>
 
+#include "blas1.hpp"
 #include "common.hpp"
 #include "context.hpp"
 
@@ -17,6 +18,8 @@ Context::Context(int device) : device_(device) {
   SU2_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   own_stream_ = true;
   pool_ = std::make_shared<DevicePool>(device, stream_);
+  // address of this device's copy of the allocation-free dot's scratch
+  noalloc_scratch_init(device);
 }
 
 Context::~Context() {

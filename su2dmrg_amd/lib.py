@@ -50,6 +50,8 @@ _SIG = {
     "su2_tensor_scal": (_i, [_p, _p, _d]),
     "su2_tensor_dot": (_i, [_p, _p, _p, _dp]),
     "su2_tensor_normalize": (_i, [_p, _p, _dp]),
+    "su2_tensor_dot_noalloc": (_i, [_p, _p, _p, _dp]),
+    "su2_noalloc_scratch_bytes": (_i, [_llp]),
     "su2_heff_create": (_i, [_p, _p, _p, _p, _i, _ip, C.POINTER(_p), _dp, _ip, C.POINTER(_p),
                              _dp, C.POINTER(_p)]),
     "su2_heff_apply": (_i, [_p, _p, _p, _p]),
@@ -257,6 +259,12 @@ class Tensor:
     def dot(self, other):
         r = _d()
         self.ctx.check(lib.su2_tensor_dot(self.ctx.h, self.h, other.h, C.byref(r)))
+        return r.value
+
+    def dot_noalloc(self, other):
+        """su2_tensor_dot_noalloc: the same dot with no allocation (fixed scratch)."""
+        r = _d()
+        self.ctx.check(lib.su2_tensor_dot_noalloc(self.ctx.h, self.h, other.h, C.byref(r)))
         return r.value
 
     def normalize(self):
