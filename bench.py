@@ -1,4 +1,4 @@
-"""Benchmark of the SU(2) H_eff matvec on one MI355X.
+"""Benchmark of the SU(2) DMRG hot path on one MI355X.
 
 A "step" is one H_eff matvec (dmrg_su2::hamiltonian_vector_multiplication_
 idmrg) over the centre bond of a 16x8 J1-J2 cylinder at D=2000 SU(2)
@@ -10,7 +10,24 @@ Inputs are resident in HBM before timing starts.
 Prints one JSON line: matvec TFLOP/s (algorithmic FLOPs in the reference's
 GEMM order / wall time), the roofline of the dominant kernel (the grouped
 FP64 MFMA GEMM, HIP-event timed on the library stream) and a CPU baseline
-(the numpy restatement on this host's cores, on a bounded sample).
+(the numpy restatement on this host's cores, on a bounded sample).  Further
+legs of the same line (each can be switched off):
+
+* fp32: the same matvec on FP32 MFMA with FP32 accumulation;
+* sweep: the real 16x8 J1-J2 (J2/J1 = 0.5) cylinder at D=2000 -- the FP64
+  run (warm-up + sweeps) and the mixed-precision run (FP32 Krylov matvecs in
+  the early sweeps with D ramping 500 -> 1000 -> 2000, FP64 for the last
+  two), with each run's sweep times and the mixed run's |dE/site| against
+  the FP64 run;
+* scan: independent J-Q-J2 DMRG jobs on 8x4 and 12x6 cylinders run one after
+  another and then all together in lockstep (su2_batch: one grouped launch
+  per GEMM stage per Lanczos round, HIP-graph replayed), jobs/hour of each;
+* serve: a Poisson stream of mixed jobs (lattices, couplings, bond
+  dimensions, ground and excited levels) through the continuous-batching
+  server, jobs/hour and p50/p99 latency against the same jobs served one at
+  a time;
+* cpu_sweep: the oracle's CPU sweep of a 16x8 cylinder at a small D beside
+  the device's sweep of the same lattice and D.
 """
 from __future__ import annotations
 
@@ -18,17 +35,32 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 MFMA
+PEAK_FP32_TFLOPS = 157.3    # MI355X dense FP32 MFMA
 PEAK_HBM_GBS = 8000.0
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1
 
 
 def measured_traffic(variants):
@@ -46,72 +78,233 @@ def measured_traffic(variants):
     return t["bytes_per_matvec"] if same else None
 
 
-def cpu_baseline(work, budget_s=20.0):
-    """Time the oracle matvec (numpy/OpenBLAS) on a bounded sample of terms."""
-    import numpy as np
+def cpu_baseline(work, budget_s=20.0, repeats=3):
+    """Oracle matvec (numpy/OpenBLAS) on the host cores: one untimed pass on a
+    bounded sample of terms, then `repeats` timed passes of that sample."""
+    import numpy as np  # noqa: F401
     from oracle import heff as oheff
     from su2dmrg_amd import synthetic
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    # one term first to size the sample
+    model, ncpu = cpu_info()
+    threads = int(os.environ.get("OMP_NUM_THREADS", ncpu))
     sub = dict(work)
     sub["terms"] = work["terms"][:1]
     t0 = time.perf_counter()
     oheff.heff_apply(work["psi"], None, None, sub["terms"])
     t1 = time.perf_counter() - t0
-    nterm = max(1, min(len(work["terms"]), int(budget_s / max(t1, 1e-3))))
+    per_pass = budget_s / max(1, repeats + 1)
+    nterm = max(1, min(len(work["terms"]), int(per_pass / max(t1, 1e-3))))
     sub["terms"] = work["terms"][:nterm]
-    t0 = time.perf_counter()
-    oheff.heff_apply(work["psi"], work["hl"], work["hr"], sub["terms"])
-    dt = time.perf_counter() - t0
+    oheff.heff_apply(work["psi"], work["hl"], work["hr"], sub["terms"])   # warm-up
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        oheff.heff_apply(work["psi"], work["hl"], work["hr"], sub["terms"])
+        times.append(time.perf_counter() - t0)
+    dt = min(times)
     f = synthetic.flops(sub)
     return {"value": f / dt / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "nproc": ncpu,
             "sample": f"oracle numpy matvec with {nterm} of {len(work['terms'])} term pairs "
-                      f"plus both block Hamiltonians, one pass ({dt:.2f} s)"}
+                      f"plus both block Hamiltonians: one warm-up pass, best of {repeats} "
+                      f"timed passes ({', '.join(f'{t:.2f}' for t in times)} s)"}
 
 
-def sweep_leg(L, ctx, lx, ly, dcut, variants):
-    """Time one full sweep of the real 16x8 J1-J2 (J2/J1 = 0.5) cylinder.
-
-    The warm-up (do_idmrg) grows the blocks at the same D and is untimed; the
-    sweep (dmrg_su2::sweep, three passes, nfree = 2) is timed end to end on
-    the host clock, after a device synchronisation on both sides.
-    """
-    dm = L.Dmrg(ctx, lx, ly, 2.0, 0.0, 2, 0, dcut)
-    dm.set_verbose(1)
-    if variants:
-        dm.set_heff_variants(*variants)
+def run_schedule(L, ctx, lx, ly, schedule, label):
+    """Warm-up then one sweep per (D, bits) entry; per-sweep wall times (device
+    synchronised on both sides), energies and the device-side breakdown."""
+    dm = L.Dmrg(ctx, lx, ly, 2.0, 0.0, 2, 0, schedule[0][0])
+    d0, b0 = schedule[0]
+    dm.set_max_dcut(d0)
+    dm.set_matvec_precision(b0)
+    ns = lx * ly
     t0 = time.perf_counter()
     e_w = dm.warmup()
     ctx.sync()
     t_warm = time.perf_counter() - t0
-    s0 = dm.stats()
-    log(f"sweep leg: warm-up {t_warm:.1f} s, E = {e_w:.10f}")
-    t0 = time.perf_counter()
-    e = dm.sweep()
-    ctx.sync()
-    t_sweep = time.perf_counter() - t0
-    s1 = dm.stats()
-    d = {k: s1[k] - s0[k] for k in s1}
+    log(f"{label}: warm-up {t_warm:.1f} s, E/site {0.5 * e_w / ns:.10f}")
+    prev = dm.stats()
+    sweeps = []
+    for k, (d, bits) in enumerate(schedule):
+        dm.set_max_dcut(d)
+        dm.set_matvec_precision(bits)
+        t0 = time.perf_counter()
+        e = dm.sweep()
+        ctx.sync()
+        ts = time.perf_counter() - t0
+        st = dm.stats()
+        dd = {x: st[x] - prev[x] for x in st}
+        prev = st
+        nstep = int(dd["steps"])
+        rec = {"max_dcut": d, "bits": bits, "time_s": round(ts, 3),
+               "energy_per_site": 0.5 * e / ns,
+               "matvecs": int(dd["matvecs"]), "fp32_matvecs": int(dd["matvecs32"]),
+               "matvec_tflops": round(dd["heff_flops"] / max(dd["t_matvec"], 1e-12) / 1e12, 3),
+               "breakdown_s": {x: round(dd[x], 3) for x in
+                               ("t_env", "t_lanczos", "t_matvec", "t_matvec32", "t_svd",
+                                "t_jacobi", "t_project", "t_start")},
+               "jacobi_sweeps_per_step": round(dd["jacobi_sweeps"] / max(nstep, 1), 2)}
+        sweeps.append(rec)
+        log(f"{label}: sweep {k + 1} D={d} {bits}-bit {ts:.1f} s, E/site {rec['energy_per_site']:.12f}"
+            f", SVD {dd['t_svd']:.1f} s, matvec {dd['t_matvec']:.1f} s")
     hist = dm.history()
-    nstep = int(d["steps"])
-    out = {
-        "lattice": f"{lx}x{ly} cylinder, J2/J1=0.5, S=0",
-        "max_dcut": dcut,
-        "sweep_time_s": round(t_sweep, 3),
-        "warmup_time_s": round(t_warm, 3),
-        "steps": nstep,
-        "energy": e,
-        "energy_per_site": 0.5 * e / (lx * ly),
-        "max_kept": max(h[0] for h in hist[-nstep:]),
-        "matvecs": int(d["matvecs"]),
-        "matvec_tflops": round(d["heff_flops"] / max(d["t_matvec"], 1e-12) / 1e12, 3),
-        "breakdown_s": {k: round(d[k], 3) for k in
-                        ("t_env", "t_lanczos", "t_matvec", "t_svd", "t_jacobi", "t_project",
-                         "t_start")},
-        "jacobi_sweeps_per_step": round(d["jacobi_sweeps"] / max(nstep, 1), 2),
-    }
+    st = dm.stats()
+    out = {"lattice": f"{lx}x{ly} cylinder, J2/J1=0.5, S=0", "schedule": schedule,
+           "warmup_time_s": round(t_warm, 3), "sweeps": sweeps,
+           "total_sweep_time_s": round(sum(s["time_s"] for s in sweeps), 3),
+           "final_energy_per_site": sweeps[-1]["energy_per_site"],
+           "max_kept": max(h[0] for h in hist),
+           "max_fp64_correction": st["max_fp64_correction"]}
     dm.close()
     return out
+
+
+def sweep_legs(L, ctx, args):
+    lx, ly = (int(x) for x in args.sweep_lattice.split("x"))
+    D = args.sweep_d
+    n = args.sweeps
+    fp64 = run_schedule(L, ctx, lx, ly, [(D, 64)] * n, "fp64 run")
+    ramp = [d for d in (500, 1000, 2000) if d < D] + [D]
+    ramp = (ramp + [D] * 3)[:max(1, n - 2)]
+    mixed_sched = [(d, 32) for d in ramp] + [(D, 64)] * min(2, n)
+    mixed = run_schedule(L, ctx, lx, ly, mixed_sched, "mixed run")
+    d_e = abs(mixed["final_energy_per_site"] - fp64["final_energy_per_site"])
+    return {"fp64": fp64, "mixed": mixed, "mixed_vs_fp64_abs_de_per_site": d_e,
+            "note": "both runs start from the reference's all-ones warm-up vector; with "
+                    f"{n} sweeps neither is converged, so |dE/site| measures the two "
+                    "schedules' different trajectories, not FP32 rounding"}
+
+
+def scan_jobs(njobs, dvals, seed=1):
+    """J-Q-J2 jobs of the parameter scan: J2/J1 uniform on [0.3, 0.7], Q/J1 on
+    [0, 2], s = J2 + Q, -jcoup 1/s -qcoup Q/s (dmrg_su2.cpp:258-260); 8x4 and
+    12x6 cylinders alternately, 2S from {0, 2, 4}, D from dvals."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(njobs):
+        j2 = rng.uniform(0.3, 0.7)
+        q = rng.uniform(0.0, 2.0)
+        s = j2 + q
+        lx, ly = ((8, 4), (12, 6))[k % 2]
+        out.append(dict(lx=lx, ly=ly, jcoup=1.0 / s, qcoup=q / s,
+                        totspin=int(rng.choice([0, 2, 4])), D=int(rng.choice(dvals))))
+    return out
+
+
+def scan_leg(L, ctx, njobs, nsweep, dvals):
+    jobs = scan_jobs(njobs, dvals)
+    sched = lambda D: [(D, 64)] * nsweep  # noqa: E731
+
+    def make(j):
+        return L.Dmrg(ctx, j["lx"], j["ly"], j["jcoup"], j["qcoup"], 2, j["totspin"], j["D"])
+
+    # one job after another (each alone on the device)
+    ctx.sync()
+    t0 = time.perf_counter()
+    seq = []
+    for j in jobs:
+        g = make(j)
+        seq.append(g.run_schedule(sched(j["D"]))[-1])
+        g.close()
+    ctx.sync()
+    t_seq = time.perf_counter() - t0
+    # all together in lockstep
+    b = L.Batch(ctx)
+    runs = []
+    t0 = time.perf_counter()
+    for j in jobs:
+        g = make(j)
+        runs.append((g, b.add(g, sched(j["D"]))))
+    b.run()
+    ctx.sync()
+    t_bat = time.perf_counter() - t0
+    same = all(b.energies(rid)[1][-1] == e for (_, rid), e in zip(runs, seq))
+    st = b.stats()
+    b.close()
+    for g, _ in runs:
+        g.close()
+    return {"jobs": njobs, "lattices": "8x4 and 12x6", "D": list(dvals),
+            "sweeps_per_job": nsweep,
+            "sequential_s": round(t_seq, 3), "batched_s": round(t_bat, 3),
+            "jobs_per_hour_sequential": round(3600 * njobs / t_seq, 1),
+            "jobs_per_hour_batched": round(3600 * njobs / t_bat, 1),
+            "speedup": round(t_seq / t_bat, 3),
+            "energies_bitwise_equal": same,
+            "batch_stats": {k: round(v, 3) for k, v in st.items()}}
+
+
+def serve_leg(L, ctx, njobs, rate_per_s, seed=3):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    lattices = [(6, 4), (8, 4), (8, 6), (12, 6), (16, 8)]
+    jobs = []
+    for k in range(njobs):
+        lx, ly = lattices[int(rng.integers(len(lattices)))]
+        j2 = rng.uniform(0.3, 0.7)
+        q = rng.uniform(0.0, 2.0)
+        s = j2 + q
+        D = int(rng.choice((64, 100, 150)))
+        exci = int(rng.choice((1, 2))) if k % 2 else 0
+        jobs.append(L.Job(lx, ly, jcoup=1.0 / s, qcoup=q / s, max_dcut=D, exci=exci, nsweep=1))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate_per_s, njobs))
+    budget = 0.8 * 288e9
+    with tempfile.TemporaryDirectory() as d:
+        srv = L.Server(ctx, budget, 16, d)
+        ids = [srv.submit(j, float(t)) for j, t in zip(jobs, arrivals)]
+        srv.run()
+        res = [srv.result(i) for i in ids]
+        st = srv.stats()
+        srv.close()
+    with tempfile.TemporaryDirectory() as d:
+        # the same stream, one job at a time (plain sequential GPU runs)
+        srv = L.Server(ctx, budget, 1, d)
+        ids = [srv.submit(j, float(t)) for j, t in zip(jobs, arrivals)]
+        srv.run()
+        res1 = [srv.result(i) for i in ids]
+        st1 = srv.stats()
+        srv.close()
+    lat = np.array([r["latency"] for r in res])
+    lat1 = np.array([r["latency"] for r in res1])
+    same = all(a["energy"] == b["energy"] for a, b in zip(res, res1))
+    return {"jobs": njobs, "poisson_rate_per_s": rate_per_s,
+            "lattices": [f"{a}x{b}" for a, b in lattices], "D": [64, 100, 150],
+            "excited_share": 0.5, "slots": 16, "budget_bytes": budget,
+            "wall_s": round(st["wall"], 3), "max_concurrent": int(st["max_concurrent"]),
+            "jobs_per_hour": round(3600 * njobs / st["wall"], 1),
+            "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
+            "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
+            "sequential": {"wall_s": round(st1["wall"], 3),
+                           "jobs_per_hour": round(3600 * njobs / st1["wall"], 1),
+                           "p50_latency_s": round(float(np.percentile(lat1, 50)), 3),
+                           "p99_latency_s": round(float(np.percentile(lat1, 99)), 3)},
+            "throughput_ratio": round(st1["wall"] / st["wall"], 3),
+            "energies_bitwise_equal_to_sequential": same}
+
+
+def cpu_sweep_leg(L, ctx, lx, ly, D):
+    """The oracle's CPU sweep beside the device's, same lattice, D and start."""
+    from oracle import dmrg as od
+    lat = od.Lattice(lx, ly, jcoup=2.0, bdry=2)
+    o = od.DMRG(lat, totspin=0, max_dcut=D)
+    o.warmup()
+    t0 = time.perf_counter()
+    e_cpu = o.sweep()
+    t_cpu = time.perf_counter() - t0
+    g = L.Dmrg(ctx, lx, ly, 2.0, 0.0, 2, 0, D)
+    g.warmup()
+    ctx.sync()
+    t0 = time.perf_counter()
+    e_gpu = g.sweep()
+    ctx.sync()
+    t_gpu = time.perf_counter() - t0
+    g.close()
+    model, ncpu = cpu_info()
+    return {"lattice": f"{lx}x{ly}", "max_dcut": D, "cpu_sweep_s": round(t_cpu, 3),
+            "gpu_sweep_s": round(t_gpu, 3), "ratio": round(t_cpu / t_gpu, 2),
+            "cpu_energy_per_site": 0.5 * e_cpu / (lx * ly),
+            "gpu_energy_per_site": 0.5 * e_gpu / (lx * ly),
+            "cpu": f"oracle (numpy/OpenBLAS, {model}, {ncpu} cores), one sweep after its own "
+                   "warm-up"}
 
 
 def main():
@@ -121,22 +314,30 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="cyl16x8_d2000",
                     choices=["cyl16x8_d2000", "cyl8x4_d1000"])
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--variants", default="",
                     help="pin the GEMM tile variants 'stage1,stage2' instead of autotuning")
-    ap.add_argument("--sweep-lattice", default="16x8",
-                    help="lattice of the timed sweep, or 'none' to skip the sweep leg")
-    ap.add_argument("--sweep-d", type=int, default=2000, help="max multiplets of the sweep leg")
-    ap.add_argument("--svd-block", type=int, default=32, choices=[16, 32],
-                    help="column block width of the Jacobi SVD")
-    ap.add_argument("--svd-inner", type=int, default=1, choices=[1, 2, 3],
-                    help="inner Jacobi sweeps per visit of a column pair")
+    ap.add_argument("--legs", default="fp32,sweep,scan,serve,cpu_sweep",
+                    help="comma list of extra legs (fp32, sweep, scan, serve, cpu_sweep) or 'none'")
+    ap.add_argument("--sweep-lattice", default="16x8")
+    ap.add_argument("--sweep-d", type=int, default=2000)
+    ap.add_argument("--sweeps", type=int, default=3,
+                    help="sweeps of each config-3 run (the mixed run: ramp, then two FP64)")
+    ap.add_argument("--scan-jobs", type=int, default=16)
+    ap.add_argument("--scan-sweeps", type=int, default=1)
+    ap.add_argument("--scan-d", default="100,200", help="bond dimensions drawn by the scan jobs")
+    ap.add_argument("--serve-jobs", type=int, default=12)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--cpu-sweep", default="16x8:64")
+    ap.add_argument("--svd-block", type=int, default=32, choices=[16, 32])
+    ap.add_argument("--svd-inner", type=int, default=1, choices=[1, 2, 3])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU sharding of the matvec is designed but not built", file=sys.stderr)
         sys.exit(2)
+    legs = set() if args.legs == "none" else set(args.legs.split(","))
 
     from su2dmrg_amd import lib as L
     from su2dmrg_amd import synthetic
@@ -179,23 +380,67 @@ def main():
     achieved = alg / (gemm_ms * 1e-3) / 1e12
     log(f"events: apply {ms_apply:.3f} ms, stage1 {ms1:.3f} ms, stage2 {ms2:.3f} ms")
 
+    fp32 = None
+    if "fp32" in legs:
+        plan.set_precision(32)
+        ch32, _, _ = plan.autotune(reps=3)
+        a32, s1, s2 = plan.time(dev["psi"], dev["out"], 10)
+        fp32 = {"variants": list(ch32), "ms_apply": round(a32, 4),
+                "ms_stage1": round(s1, 4), "ms_stage2": round(s2, 4),
+                "gemm_tflops": round(alg / ((s1 + s2) * 1e-3) / 1e12, 3),
+                "pct_fp32_peak": round(100 * alg / ((s1 + s2) * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 2),
+                "kernel": "grouped_gemm_f64_uniform<float,...> (v_mfma_f32_16x16x4_f32, FP32 "
+                          "accumulation)"}
+        log(f"fp32 matvec: {fp32['gemm_tflops']} TFLOP/s ({fp32['pct_fp32_peak']}% of FP32 peak)")
+        plan.set_precision(64)
+    for k in ("psi", "out", "hl", "hr", "plan"):
+        if dev.get(k) is not None:
+            dev[k].close()
+    for t in dev["ops"]:
+        t.close()
+
     sweep = None
-    if args.sweep_lattice != "none":
-        lx, ly = (int(x) for x in args.sweep_lattice.split("x"))
-        log(f"sweep leg: {lx}x{ly} at D={args.sweep_d}")
+    if "sweep" in legs:
         try:
-            sweep = sweep_leg(L, ctx, lx, ly, args.sweep_d, None)
-            log(f"sweep leg: {sweep['sweep_time_s']:.1f} s, E/site {sweep['energy_per_site']:.10f}")
+            sweep = sweep_legs(L, ctx, args)
         except L.Su2Error as exc:
-            # the matvec figures above stand on their own; record why the sweep did not
-            log(f"sweep leg failed: {exc}")
-            sweep = {"lattice": f"{lx}x{ly}", "max_dcut": args.sweep_d, "error": str(exc)}
+            log(f"sweep legs failed: {exc}")
+            sweep = {"error": str(exc)}
+    scan = None
+    if "scan" in legs:
+        log(f"scan: {args.scan_jobs} jobs, sequential then batched")
+        scan = scan_leg(L, ctx, args.scan_jobs, args.scan_sweeps,
+                        [int(x) for x in args.scan_d.split(",")])
+        log(f"scan: {scan['jobs_per_hour_sequential']} -> {scan['jobs_per_hour_batched']} jobs/h "
+            f"({scan['speedup']}x), bitwise equal: {scan['energies_bitwise_equal']}")
+    serve = None
+    if "serve" in legs:
+        log(f"serve: {args.serve_jobs} jobs at {args.serve_rate}/s")
+        serve = serve_leg(L, ctx, args.serve_jobs, args.serve_rate)
+        log(f"serve: {serve['jobs_per_hour']} jobs/h, p50 {serve['p50_latency_s']} s, p99 "
+            f"{serve['p99_latency_s']} s; sequential {serve['sequential']['jobs_per_hour']} jobs/h")
 
     cpu = None
+    cpu_sweep = None
     if not args.no_cpu:
-        log("cpu baseline (oracle) ...")
+        log("cpu baseline (oracle matvec) ...")
         cpu = cpu_baseline(work, args.cpu_budget)
         log(f"cpu baseline {cpu['value']:.3f} TFLOP/s on {cpu['cores']} threads")
+        if "cpu_sweep" in legs:
+            lat, dcut = args.cpu_sweep.split(":")
+            lx, ly = (int(x) for x in lat.split("x"))
+            log(f"cpu sweep: oracle vs device, {lat} at D={dcut}")
+            cpu_sweep = cpu_sweep_leg(L, ctx, lx, ly, int(dcut))
+            log(f"cpu sweep {cpu_sweep['cpu_sweep_s']} s vs device {cpu_sweep['gpu_sweep_s']} s")
+        if sweep and "fp64" in sweep:
+            # the headline sweep on the CPU cannot finish in the bench; its GEMM
+            # work alone, at the measured CPU matvec rate, bounds it from below
+            s0 = sweep["fp64"]["sweeps"][0]
+            tflop = s0["matvec_tflops"] * s0["breakdown_s"]["t_matvec"]
+            cpu["sweep_lower_bound_s"] = round(tflop / cpu["value"], 1)
+            cpu["sweep_lower_bound_note"] = (
+                "the first FP64 16x8 D=2000 sweep's algorithmic matvec FLOPs divided by the "
+                "CPU matvec rate above (matvec GEMMs only; no SVD, environments or copies)")
 
     line = {
         "metric": "Heff matvec TFLOP/s (% FP64 MFMA peak) + sweep time, 16x8 J1-J2 D=2000",
@@ -216,12 +461,17 @@ def main():
                         f"{len(work['terms'])} spin-1 term pairs + hL + hR, "
                         f"sectors 2j={work['twoj']} dims={work['dims_a']}",
             "algorithmic_gflop_per_step": round(alg / 1e9, 3),
-            "sweep_time_s": None if sweep is None else sweep.get("sweep_time_s"),
-            "sweep": sweep,
+            "sweep_time_s": None if not sweep or "fp64" not in sweep
+            else sweep["fp64"]["sweeps"][0]["time_s"],
             "gemm_variants": list(chosen),
             "svd_block_cols": args.svd_block,
             "svd_inner_sweeps": args.svd_inner,
         },
+        "fp32_matvec": fp32,
+        "sweep": sweep,
+        "scan": scan,
+        "serve": serve,
+        "cpu_sweep": cpu_sweep,
         "roofline": {
             "bound": "mfma",
             "achieved": round(achieved, 4),

@@ -121,18 +121,25 @@ int su2_noalloc_scratch_bytes(long long* bytes);
  * dmrg_su2.cpp:543-664, and tensor_su2::hamiltonian_vector_multiplication,
  * tensor_su2_dmrg_src.cpp:893-1041).
  *   psi_layout : two-leg singlet wavefunction defining the block layout;
- *   hl, hr     : block Hamiltonians hh[il], hh[ir] (nullable);
- *   nterm      : number of S_L . S_R terms;
+ *   hl, hr     : block Hamiltonians hh[il], hh[ir] (nullable); the plan
+ *                copies them, so they may be destroyed after this call;
+ *   nterm      : number of L . R terms;
+ *   term_rank  : doubled rank 2K of each term's operator leg (nullable: all
+ *                2, the spin operators of dmrg_su2.cpp:585-610; 0 and 4 are
+ *                the rank-0 and rank-2 parts of the J-Q plaquette terms,
+ *                dmrg_su2.cpp:635-655), contracted with fac_hamilt_vec[j][2K][j'];
  *   left_count[t] operators of term t are summed as
  *     sum_i left_coef[i] * left_ops[i] (consecutive across terms), likewise
  *     for the right side -- this is the coupling sum op = sum_k coup *
  *     opr[ir][k] of dmrg_su2.cpp:585-599, formed once here.
- * Operators are three-leg [ket, spin-1 leg (2j=2, one multiplet), bra]. */
+ * Operators are three-leg [ket, operator leg (2j = term_rank[t], one
+ * multiplet), bra]; the plan copies what it needs from them. */
 int su2_heff_create(su2_ctx* ctx, const su2_tensor* psi_layout, const su2_tensor* hl,
-                    const su2_tensor* hr, int nterm, const int* left_count,
-                    const su2_tensor* const* left_ops, const double* left_coef,
-                    const int* right_count, const su2_tensor* const* right_ops,
-                    const double* right_coef, su2_heff** out);
+                    const su2_tensor* hr, int nterm, const int* term_rank,
+                    const int* left_count, const su2_tensor* const* left_ops,
+                    const double* left_coef, const int* right_count,
+                    const su2_tensor* const* right_ops, const double* right_coef,
+                    su2_heff** out);
 /* out = H_eff psi (psi, out: the plan's layout; out may not alias psi). */
 int su2_heff_apply(su2_ctx* ctx, su2_heff* plan, const su2_tensor* psi, su2_tensor* out);
 /* Algorithmic FP64 flops of one apply in the reference's GEMM order
@@ -187,6 +194,29 @@ int su2_heff_set_kernel_path(su2_ctx* ctx, su2_heff* plan, int uniform, int* lay
 int su2_svd_truncate(su2_ctx* ctx, const su2_tensor* psi, double p1, double p2, int max_dcut,
                      su2_tensor** U, su2_tensor** V, double* kept, int cap, int* nkept,
                      double* discarded);
+
+/* One environment kernel on one block operator: the tensor_su2 members
+ * that dmrg_su2::prepare_site_operator_from_left/right (dmrg_su2_omp.cpp:
+ * 319-825) call, tensor_su2_dmrg_src.cpp:27-890.  The block leg (nsec
+ * sectors: twoj ascending, multiplet counts) is enlarged by one spin-1/2 site
+ * (side 0: left block, block x site; 1: right block, site x block); the
+ * kernel acts there, then the result is projected, O' = P^T O P, when P is
+ * given -- P = [enlarged leg, new bond], the U (left) or V (right) that
+ * su2_svd_truncate returns for the step (the reference's uu[i]); with P NULL
+ * the result stays on the enlarged leg (the reference's uu1, the fusion
+ * isometry, dmrg_su2_omp.cpp:604-825).
+ *   kind 0 operator_transformation  S_k (x) 1     op: 3-leg spin operator
+ *   kind 1 overlap_transformation   H (x) 1       op: 2-leg block Hamiltonian
+ *   kind 2 operator_pairup          coef S_k . s  op: 3-leg spin operator,
+ *                                                 coef: the coupling J
+ *   kind 3 operator_initial         1 (x) s       op: NULL
+ * The recoupling factors are the fac_operator_* tables (dmrg_su2.cpp:
+ * 759-943) and the site operator carries the reference's sigma = +-sqrt(3)/2
+ * (dmrg_su2.cpp:503-506).  *out: a new tensor (3 legs for kinds 0 and 3, 2
+ * legs for 1 and 2). */
+int su2_env_transform(su2_ctx* ctx, int kind, int side, int nsec, const int* twoj,
+                      const int* dims, const su2_tensor* P, const su2_tensor* op, double coef,
+                      su2_tensor** out);
 
 /* Recoupling scalars of the environment kernels for one label tuple (doubled
  * spins), the entries of the reference's fac_* tables (dmrg_su2.cpp:759-943):

@@ -308,10 +308,11 @@ int su2_tensor_normalize(su2_ctx* ctx, su2_tensor* x, double* norm) {
 }
 
 int su2_heff_create(su2_ctx* ctx, const su2_tensor* psi_layout, const su2_tensor* hl,
-                    const su2_tensor* hr, int nterm, const int* left_count,
-                    const su2_tensor* const* left_ops, const double* left_coef,
-                    const int* right_count, const su2_tensor* const* right_ops,
-                    const double* right_coef, su2_heff** out) {
+                    const su2_tensor* hr, int nterm, const int* term_rank,
+                    const int* left_count, const su2_tensor* const* left_ops,
+                    const double* left_coef, const int* right_count,
+                    const su2_tensor* const* right_ops, const double* right_coef,
+                    su2_heff** out) {
   return guard(ctx, [&] {
     need(ctx, "ctx");
     need(psi_layout, "psi_layout");
@@ -330,6 +331,7 @@ int su2_heff_create(su2_ctx* ctx, const su2_tensor* psi_layout, const su2_tensor
     for (int t = 0; t < nterm; ++t) {
       SU2_CHECK(left_count[t] >= 1 && right_count[t] >= 1, su2::kErrArgument,
                 "term %d: empty side", t);
+      terms[t].k = term_rank ? term_rank[t] : 2;
       for (int i = 0; i < left_count[t]; ++i, ++li) {
         need(left_ops[li], "left operator");
         terms[t].left.push_back({left_coef[li], &left_ops[li]->t});
@@ -467,6 +469,23 @@ int su2_heff_set_kernel_path(su2_ctx* ctx, su2_heff* plan, int uniform, int* lay
       layouts[0] = plan->p.stage(0).layout();
       layouts[1] = plan->p.stage(1).layout();
     }
+  });
+}
+
+int su2_env_transform(su2_ctx* ctx, int kind, int side, int nsec, const int* twoj,
+                      const int* dims, const su2_tensor* P, const su2_tensor* op, double coef,
+                      su2_tensor** out) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(out, "out");
+    SU2_CHECK(side == 0 || side == 1, su2::kErrArgument, "env transform: side %d", side);
+    SU2_CHECK(nsec >= 1 && twoj && dims, su2::kErrArgument, "env transform: block leg");
+    su2::Leg block = su2::make_leg(std::vector<int>(twoj, twoj + nsec),
+                                   std::vector<int>(dims, dims + nsec), 1);
+    std::unique_ptr<su2::Tensor> t =
+        su2::env_transform(ctx->c, kind, side == 0, block, P ? &P->t : nullptr,
+                           op ? &op->t : nullptr, coef);
+    *out = new su2_tensor(t.release());
   });
 }
 

@@ -92,6 +92,122 @@ Dmrg::Dmrg(Context& c, int lx, int ly, double jcoup, double qcoup, int bdry, int
 Fused Dmrg::fused_left(int i) const { return make_fused(lb_.at(i - 1), true); }
 Fused Dmrg::fused_right(int i) const { return make_fused(rb_.at(i + 1), false); }
 
+// The four environment kernels on the enlarged basis E (the reference's
+// uu1 = fusion isometry, dmrg_su2_omp.cpp:604-825), as strided scale-adds
+// with the recoupling factors; each adds into its output in call order.
+
+// H (x) 1: overlap_transformation, factor 1 (tensor_su2_dmrg_src.cpp:531-688)
+void env_overlap(ScatterBatch& sb, const Fused& E, const Tensor& H, Tensor& Hh) {
+  for (int jf : E.fused.twoj) {
+    const int bh = Hh.find2(jf, jf);
+    const long long Df = E.dim(jf);
+    for (auto& p : E.parts(jf)) {
+      const int jb = std::get<0>(p);
+      const long long o = std::get<1>(p);
+      const int d = std::get<2>(p);
+      const int bs = H.find2(jb, jb);
+      if (bs < 0 || bh < 0) continue;
+      StridedAxpy a{};
+      a.dst = Hh.data() + Hh.blocks()[bh].offset + o + o * Df;
+      a.src = H.data() + H.blocks()[bs].offset;
+      a.rs = 1;
+      a.cs = Df;
+      a.ld = d;
+      a.rows = d;
+      a.cols = d;
+      a.alpha = 1.0;
+      sb.add(a.dst, a);
+    }
+  }
+}
+
+// coef S_k . s: operator_pairup (tensor_su2_dmrg_src.cpp:691-890), with
+// sigma[1] on the left and sigma[0] on the right (dmrg_su2_omp.cpp:645,757)
+void env_pairup(ScatterBatch& sb, const Fused& E, const Tensor& Sk, double coef, bool left,
+                Tensor& Hh) {
+  const double sig_pair = left ? kSigmaRight : kSigmaLeft;
+  for (int jf : E.fused.twoj) {
+    const int bh = Hh.find2(jf, jf);
+    if (bh < 0) continue;
+    const long long Df = E.dim(jf);
+    auto parts = E.parts(jf);
+    for (auto& p : parts)
+      for (auto& q : parts) {
+        const int jb = std::get<0>(p), jbp = std::get<0>(q);
+        const double f = left ? rc_pairup_left(jf, jb, jbp, kOp, kPhys, kPhys)
+                              : rc_pairup_right(jf, jb, jbp, kOp, kPhys, kPhys);
+        if (std::fabs(f) < 1e-8) continue;
+        const int bs = Sk.find3(jb, kOp, jbp);
+        if (bs < 0) continue;
+        StridedAxpy a{};
+        a.dst = Hh.data() + Hh.blocks()[bh].offset + std::get<1>(p) + std::get<1>(q) * Df;
+        a.src = Sk.data() + Sk.blocks()[bs].offset;
+        a.rs = 1;
+        a.cs = Df;
+        a.ld = std::get<2>(p);
+        a.rows = std::get<2>(p);
+        a.cols = std::get<2>(q);
+        a.alpha = coef * f * sig_pair;
+        sb.add(a.dst, a);
+      }
+  }
+}
+
+// S_k (x) 1: operator_transformation (tensor_su2_dmrg_src.cpp:221-402)
+void env_op_transform(ScatterBatch& sb, const Fused& E, const Tensor& Sk, bool left, Tensor& T) {
+  for (auto& blk : T.blocks()) {
+    const int jf = blk.twoj[0], jfp = blk.twoj[2];
+    const long long Df = E.dim(jf);
+    for (auto& p : E.parts(jf))
+      for (auto& q : E.parts(jfp)) {
+        const int jb = std::get<0>(p), jbp = std::get<0>(q);
+        const double f = left ? rc_op_transform_left(jb, jf, jbp, jfp, kOp, kPhys)
+                              : rc_op_transform_right(jb, jf, jbp, jfp, kOp, kPhys);
+        if (std::fabs(f) < 1e-8) continue;
+        const int bs = Sk.find3(jb, kOp, jbp);
+        if (bs < 0) continue;
+        StridedAxpy a{};
+        a.dst = T.data() + blk.offset + std::get<1>(p) + std::get<1>(q) * Df;
+        a.src = Sk.data() + Sk.blocks()[bs].offset;
+        a.rs = 1;
+        a.cs = Df;
+        a.ld = std::get<2>(p);
+        a.rows = std::get<2>(p);
+        a.cols = std::get<2>(q);
+        a.alpha = f;
+        sb.add(a.dst, a);
+      }
+  }
+}
+
+// 1 (x) s: operator_initial with sigma[0] left / sigma[1] right
+// (tensor_su2_dmrg_src.cpp:27-218)
+void env_op_initial(ScatterBatch& sb, const Fused& E, bool left, Tensor& T) {
+  const double sig = left ? kSigmaLeft : kSigmaRight;
+  for (auto& blk : T.blocks()) {
+    const int jf = blk.twoj[0], jfp = blk.twoj[2];
+    const long long Df = E.dim(jf);
+    for (auto& p : E.parts(jf))
+      for (auto& q : E.parts(jfp)) {
+        if (std::get<0>(p) != std::get<0>(q)) continue;
+        const int jb = std::get<0>(p);
+        const double f = left ? rc_op_initial_left(jb, jf, jfp, kOp, kPhys, kPhys)
+                              : rc_op_initial_right(jb, jf, jfp, kOp, kPhys, kPhys);
+        if (std::fabs(f) < 1e-8) continue;
+        StridedAxpy a{};
+        a.dst = T.data() + blk.offset + std::get<1>(p) + std::get<1>(q) * Df;
+        a.src = nullptr;     // identity on the block sector
+        a.rs = Df + 1;
+        a.cs = 0;
+        a.ld = 0;
+        a.rows = std::get<2>(p);
+        a.cols = 1;
+        a.alpha = f * sig;
+        sb.add(a.dst, a);
+      }
+  }
+}
+
 std::unique_ptr<Env> Dmrg::enlarge(const Fused& E, const Env& env, const std::vector<int>& keep,
                                    const std::vector<std::pair<double, int>>& pair, int newsite,
                                    bool left) {
@@ -103,120 +219,18 @@ std::unique_ptr<Env> Dmrg::enlarge(const Fused& E, const Env& env, const std::ve
   out->S[newsite] = vector_op(c, E.fused);
   ScatterBatch sb;
   Tensor& Hh = *out->H;
-  // H (x) 1: overlap_transformation, factor 1 (tensor_su2_dmrg_src.cpp:531-688)
-  if (env.H) {
-    for (int jf : E.fused.twoj) {
-      const int bh = Hh.find2(jf, jf);
-      const long long Df = E.dim(jf);
-      for (auto& p : E.parts(jf)) {
-        const int jb = std::get<0>(p);
-        const long long o = std::get<1>(p);
-        const int d = std::get<2>(p);
-        const int bs = env.H->find2(jb, jb);
-        if (bs < 0 || bh < 0) continue;
-        StridedAxpy a{};
-        a.dst = Hh.data() + Hh.blocks()[bh].offset + o + o * Df;
-        a.src = env.H->data() + env.H->blocks()[bs].offset;
-        a.rs = 1;
-        a.cs = Df;
-        a.ld = d;
-        a.rows = d;
-        a.cols = d;
-        a.alpha = 1.0;
-        sb.add(a.dst, a);
-      }
-    }
-  }
-  // sum_k J S_k . s : operator_pairup (tensor_su2_dmrg_src.cpp:691-890), with
-  // sigma[1] on the left and sigma[0] on the right (dmrg_su2_omp.cpp:645,757)
-  const double sig_pair = left ? kSigmaRight : kSigmaLeft;
+  if (env.H) env_overlap(sb, E, *env.H, Hh);
   for (auto& pr : pair) {
     auto it = env.S.find(pr.second);
     SU2_CHECK(it != env.S.end(), kErrStructure, "pair operator of site %d missing", pr.second);
-    const Tensor& Sk = *it->second;
-    for (int jf : E.fused.twoj) {
-      const int bh = Hh.find2(jf, jf);
-      if (bh < 0) continue;
-      const long long Df = E.dim(jf);
-      auto parts = E.parts(jf);
-      for (auto& p : parts)
-        for (auto& q : parts) {
-          const int jb = std::get<0>(p), jbp = std::get<0>(q);
-          const double f = left ? rc_pairup_left(jf, jb, jbp, kOp, kPhys, kPhys)
-                                : rc_pairup_right(jf, jb, jbp, kOp, kPhys, kPhys);
-          if (std::fabs(f) < 1e-8) continue;
-          const int bs = Sk.find3(jb, kOp, jbp);
-          if (bs < 0) continue;
-          StridedAxpy a{};
-          a.dst = Hh.data() + Hh.blocks()[bh].offset + std::get<1>(p) + std::get<1>(q) * Df;
-          a.src = Sk.data() + Sk.blocks()[bs].offset;
-          a.rs = 1;
-          a.cs = Df;
-          a.ld = std::get<2>(p);
-          a.rows = std::get<2>(p);
-          a.cols = std::get<2>(q);
-          a.alpha = pr.first * f * sig_pair;
-          sb.add(a.dst, a);
-        }
-    }
+    env_pairup(sb, E, *it->second, pr.first, left, Hh);
   }
-  // S_k (x) 1: operator_transformation (tensor_su2_dmrg_src.cpp:221-402)
   for (int k : keep) {
     auto it = env.S.find(k);
     SU2_CHECK(it != env.S.end(), kErrStructure, "operator of site %d missing", k);
-    const Tensor& Sk = *it->second;
-    Tensor& T = *out->S[k];
-    for (auto& blk : T.blocks()) {
-      const int jf = blk.twoj[0], jfp = blk.twoj[2];
-      const long long Df = E.dim(jf);
-      for (auto& p : E.parts(jf))
-        for (auto& q : E.parts(jfp)) {
-          const int jb = std::get<0>(p), jbp = std::get<0>(q);
-          const double f = left ? rc_op_transform_left(jb, jf, jbp, jfp, kOp, kPhys)
-                                : rc_op_transform_right(jb, jf, jbp, jfp, kOp, kPhys);
-          if (std::fabs(f) < 1e-8) continue;
-          const int bs = Sk.find3(jb, kOp, jbp);
-          if (bs < 0) continue;
-          StridedAxpy a{};
-          a.dst = T.data() + blk.offset + std::get<1>(p) + std::get<1>(q) * Df;
-          a.src = Sk.data() + Sk.blocks()[bs].offset;
-          a.rs = 1;
-          a.cs = Df;
-          a.ld = std::get<2>(p);
-          a.rows = std::get<2>(p);
-          a.cols = std::get<2>(q);
-          a.alpha = f;
-          sb.add(a.dst, a);
-        }
-    }
+    env_op_transform(sb, E, *it->second, left, *out->S[k]);
   }
-  // 1 (x) s: operator_initial with sigma[0] left / sigma[1] right (tensor_su2_dmrg_src.cpp:27-218)
-  {
-    const double sig = left ? kSigmaLeft : kSigmaRight;
-    Tensor& T = *out->S[newsite];
-    for (auto& blk : T.blocks()) {
-      const int jf = blk.twoj[0], jfp = blk.twoj[2];
-      const long long Df = E.dim(jf);
-      for (auto& p : E.parts(jf))
-        for (auto& q : E.parts(jfp)) {
-          if (std::get<0>(p) != std::get<0>(q)) continue;
-          const int jb = std::get<0>(p);
-          const double f = left ? rc_op_initial_left(jb, jf, jfp, kOp, kPhys, kPhys)
-                                : rc_op_initial_right(jb, jf, jfp, kOp, kPhys, kPhys);
-          if (std::fabs(f) < 1e-8) continue;
-          StridedAxpy a{};
-          a.dst = T.data() + blk.offset + std::get<1>(p) + std::get<1>(q) * Df;
-          a.src = nullptr;     // identity on the block sector
-          a.rs = Df + 1;
-          a.cs = 0;
-          a.ld = 0;
-          a.rows = std::get<2>(p);
-          a.cols = 1;
-          a.alpha = f * sig;
-          sb.add(a.dst, a);
-        }
-    }
-  }
+  env_op_initial(sb, E, left, *out->S[newsite]);
   sb.run(c);
   grow_plaquettes(E, env, newsite, left, *out);
   return out;
@@ -244,29 +258,11 @@ std::unique_ptr<Env> Dmrg::enlarged_right(int i, const Fused& E) {
   return enlarge(E, env, keep, pair, i, false);
 }
 
-std::unique_ptr<Env> Dmrg::project(const Tensor& P, const Env& hat) {
-  Context& c = c_;
-  const double t0 = now();
-  std::unique_ptr<Env> out(new Env());
-  out->bond = P.leg(1);
-  std::vector<std::pair<const Tensor*, Tensor*>> ops;
-  if (hat.H) {
-    out->H = scalar_op(c, out->bond);
-    ops.push_back({hat.H.get(), out->H.get()});
-  }
-  for (auto& kv : hat.S) {
-    out->S[kv.first] = vector_op(c, out->bond);
-    ops.push_back({kv.second.get(), out->S[kv.first].get()});
-  }
-  for (auto& kv : hat.P)
-    for (auto& r : kv.second) {
-      auto& dst = out->P[kv.first][r.first];
-      dst.reset(new Tensor(&c, {with_dir(out->bond, 1), make_leg({r.first}, {1}, 1),
-                                with_dir(out->bond, -1)},
-                           0, nullptr));
-      ops.push_back({r.second.get(), dst.get()});
-    }
-  // Y = O[a->b] P[b], then O'[a->b] = P[a]^T Y
+// O' = P^T O P for every (O, destination) pair (the truncated-basis update of
+// dmrg_su2_omp.cpp:319-601): Y = O[a->b] P[b], then O'[a->b] = P[a]^T Y, two
+// grouped launches for all operators.
+void project_ops(Context& c, const Tensor& P,
+                 const std::vector<std::pair<const Tensor*, Tensor*>>& ops) {
   struct Job {
     const double* O;
     const double* Pa;
@@ -300,19 +296,98 @@ std::unique_ptr<Env> Dmrg::project(const Tensor& P, const Env& hat) {
       jobs.push_back(j);
     }
   }
-  double* Y = c.dev_alloc_n<double>(ytot > 0 ? ytot : 0);
+  DevBuf Y(c, ytot);
   {
     GemmBatch g;
-    for (auto& j : jobs) g.add(Y + j.y, j.Da, j.Da, j.db, j.O, j.Da, 0, j.Pb, j.Db, 0, j.Db);
+    for (auto& j : jobs) g.add(Y.p + j.y, j.Da, j.Da, j.db, j.O, j.Da, 0, j.Pb, j.Db, 0, j.Db);
     g.run(c);
   }
   {
     GemmBatch g;
-    for (auto& j : jobs) g.add(j.dst, j.da, j.da, j.db, j.Pa, j.Da, 1, Y + j.y, j.Da, 0, j.Da);
+    for (auto& j : jobs) g.add(j.dst, j.da, j.da, j.db, j.Pa, j.Da, 1, Y.p + j.y, j.Da, 0, j.Da);
     g.run(c);
   }
-  c.dev_free(Y);
+}
+
+std::unique_ptr<Env> Dmrg::project(const Tensor& P, const Env& hat) {
+  Context& c = c_;
+  const double t0 = now();
+  std::unique_ptr<Env> out(new Env());
+  out->bond = P.leg(1);
+  std::vector<std::pair<const Tensor*, Tensor*>> ops;
+  if (hat.H) {
+    out->H = scalar_op(c, out->bond);
+    ops.push_back({hat.H.get(), out->H.get()});
+  }
+  for (auto& kv : hat.S) {
+    out->S[kv.first] = vector_op(c, out->bond);
+    ops.push_back({kv.second.get(), out->S[kv.first].get()});
+  }
+  for (auto& kv : hat.P)
+    for (auto& r : kv.second) {
+      auto& dst = out->P[kv.first][r.first];
+      dst.reset(new Tensor(&c, {with_dir(out->bond, 1), make_leg({r.first}, {1}, 1),
+                                with_dir(out->bond, -1)},
+                           0, nullptr));
+      ops.push_back({r.second.get(), dst.get()});
+    }
+  project_ops(c, P, ops);
   stats_.t_project += now() - t0;
+  return out;
+}
+
+// One environment kernel on one operator, alone (the C ABI's
+// su2_env_transform): the kernel on the enlarged basis of `block`, then the
+// projection with P when given.
+std::unique_ptr<Tensor> env_transform(Context& c, int kind, bool left, const Leg& block,
+                                      const Tensor* P, const Tensor* op, double coef) {
+  c.make_current();
+  Fused E = make_fused(block, left);
+  if (P) {
+    SU2_CHECK(P->nleg() == 2 && P->leg(0).same_sectors(E.fused), kErrStructure,
+              "env transform: projector leg 0 is not the enlarged basis of the block");
+  }
+  auto check_op = [&](int nleg) {
+    SU2_CHECK(op != nullptr, kErrArgument, "env transform: operator missing");
+    SU2_CHECK(op->nleg() == nleg, kErrStructure, "env transform: operator with %d legs",
+              op->nleg());
+    SU2_CHECK(op->leg(0).same_sectors(block) && op->leg(nleg - 1).same_sectors(block),
+              kErrStructure, "env transform: operator legs are not the block leg");
+    if (nleg == 3)
+      SU2_CHECK(op->leg(1).nsec() == 1 && op->leg(1).twoj[0] == kOp && op->leg(1).dim[0] == 1,
+                kErrStructure, "env transform: operator leg 1 must be the spin-1 multiplet");
+  };
+  const bool scalar = kind == kEnvOverlap || kind == kEnvPairup;
+  std::unique_ptr<Tensor> X = scalar ? scalar_op(c, E.fused) : vector_op(c, E.fused);
+  ScatterBatch sb;
+  switch (kind) {
+    case kEnvOpTransform:
+      check_op(3);
+      env_op_transform(sb, E, *op, left, *X);
+      break;
+    case kEnvOverlap:
+      check_op(2);
+      env_overlap(sb, E, *op, *X);
+      break;
+    case kEnvPairup:
+      check_op(3);
+      env_pairup(sb, E, *op, coef, left, *X);
+      break;
+    case kEnvInitial:
+      env_op_initial(sb, E, left, *X);
+      break;
+    default:
+      fail(kErrArgument, "env transform: kind %d", kind);
+  }
+  sb.run(c);
+  if (!P) {
+    SU2_HIP(hipStreamSynchronize(c.stream()));
+    return X;
+  }
+  const Leg bond = P->leg(1);
+  std::unique_ptr<Tensor> out = scalar ? scalar_op(c, bond) : vector_op(c, bond);
+  project_ops(c, *P, {{X.get(), out.get()}});
+  SU2_HIP(hipStreamSynchronize(c.stream()));
   return out;
 }
 

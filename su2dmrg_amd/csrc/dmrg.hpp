@@ -63,6 +63,20 @@ struct StepRecord {
   int matvecs = 0;
 };
 
+// One environment kernel applied to one block operator (the reference's
+// tensor_su2 members called by dmrg_su2::prepare_site_operator_from_left /
+// _right, dmrg_su2_omp.cpp:319-825, tensor_su2_dmrg_src.cpp:27-890), on the
+// enlarged basis of `block` with one spin-1/2 site (left: block x site,
+// right: site x block), then projected on P ([enlarged, new bond]: U or V of
+// the truncated SVD) when P is given:
+//   kEnvOpTransform  S_k (x) 1       operator_transformation   (op: 3 legs)
+//   kEnvOverlap      H (x) 1         overlap_transformation    (op: 2 legs)
+//   kEnvPairup       coef S_k . s    operator_pairup           (op: 3 legs)
+//   kEnvInitial      1 (x) s         operator_initial          (no op)
+enum EnvKind { kEnvOpTransform = 0, kEnvOverlap = 1, kEnvPairup = 2, kEnvInitial = 3 };
+std::unique_ptr<Tensor> env_transform(Context& c, int kind, bool left, const Leg& block,
+                                      const Tensor* P, const Tensor* op, double coef);
+
 // Fused leg of a block with one spin-1/2 site: block x site (left) or
 // site x block (right), with the offset of every block sector inside the
 // fused sectors (tensor_su2::fuse(b1,b2), tensor_su2.cpp:839-909).

@@ -690,9 +690,9 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
   // Row slices: a block with few pairs in a round spreads each pair over up
   // to MAX_SLICES workgroups (at least 64 rows each) so that a round fills the
   // chip.  The slice count of a pair follows from its own block (its pairs in
-  // the round, about a quarter of slice_target workgroups per block), never
-  // from the other blocks of the batch: the slices' partial Gram matrices
-  // are summed in slice order, so the slicing decides the bits.
+  // the round against slice_target workgroups), never from the other blocks
+  // of the batch: the slices' partial Gram matrices are summed in slice
+  // order, so the slicing decides the bits.
   std::vector<int> round_off(rounds.size() + 1, 0), round_pmax(rounds.size(), 1);
   std::vector<PairJob> all;
   size_t gpart_max = 0;
@@ -703,7 +703,7 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
     for (const PairJob& j : rounds[r]) per_block[j.blk]++;
     int used = 1;
     for (PairJob& j : rounds[r]) {
-      const int pmax = std::min(MAX_SLICES, std::max(1, slice_target / (4 * per_block[j.blk])));
+      const int pmax = std::min(MAX_SLICES, std::max(1, slice_target / per_block[j.blk]));
       j.P = std::max(1, std::min(pmax, L[j.blk] / MIN_SLICE_ROWS));
       used = std::max(used, j.P);
     }

@@ -25,7 +25,9 @@ double job_footprint(const JobSpec& s) {
   const double D = s.max_dcut, De = 2.0 * s.max_dcut;
   const double ns = (double)s.lx * s.ly;
   const double ops = s.ly + 2 + (s.qcoup != 0 ? 2 * s.ly : 0);
-  const double env = ns * ops * D * D;                 // environments of every site
+  // left and right environments of every site, ~0.8 D^2 doubles per operator
+  // (measured: the 16x8 D=2000 run peaks at 75 GB, this gives 82 GB)
+  const double env = 2.0 * ns * ops * 0.8 * D * D;
   const double plan = 2.0 * ops * 3.0 * De * De / 4;   // H_eff bundles (both sides)
   const double krylov = 10.0 * De * De / 4;            // Lanczos vectors, SVD work space
   const double stages = s.exci > 0 ? 1.0 + 0.5 * s.exci : 1.0;   // lower states held

@@ -43,10 +43,10 @@ struct JobResult {
   bool done = false;
 };
 
-// Estimated device bytes of a job: environments of every site (block
-// Hamiltonian and the retained spin operators, about ly + 2 operators of
-// ~D^2 doubles each), the two-site H_eff bundles and Lanczos vectors of a
-// step (enlarged dimension 2D), and SVD work space.
+// Estimated device bytes of a job: left and right environments of every
+// site (block Hamiltonian and the retained spin operators, about ly + 2
+// operators of ~0.8 D^2 doubles each), the two-site H_eff bundles and Lanczos
+// vectors of a step (enlarged dimension 2D), and SVD work space.
 double job_footprint(const JobSpec& s);
 
 class Server {

@@ -52,8 +52,9 @@ _SIG = {
     "su2_tensor_normalize": (_i, [_p, _p, _dp]),
     "su2_tensor_dot_noalloc": (_i, [_p, _p, _p, _dp]),
     "su2_noalloc_scratch_bytes": (_i, [_llp]),
-    "su2_heff_create": (_i, [_p, _p, _p, _p, _i, _ip, C.POINTER(_p), _dp, _ip, C.POINTER(_p),
-                             _dp, C.POINTER(_p)]),
+    "su2_heff_create": (_i, [_p, _p, _p, _p, _i, _ip, _ip, C.POINTER(_p), _dp, _ip,
+                             C.POINTER(_p), _dp, C.POINTER(_p)]),
+    "su2_env_transform": (_i, [_p, _i, _i, _i, _ip, _ip, _p, _p, _d, C.POINTER(_p)]),
     "su2_heff_apply": (_i, [_p, _p, _p, _p]),
     "su2_heff_flops": (_i, [_p, _dp, _dp]),
     "su2_heff_destroy": (_i, [_p, _p]),
@@ -307,14 +308,14 @@ def scalar_operator(ctx, twoj, dims, h):
     return t
 
 
-def vector_operator(ctx, twoj, dims, op):
-    """Three-leg [ket, spin-1 leg, bra] operator holding the blocks in ``op``."""
+def vector_operator(ctx, twoj, dims, op, rank=2):
+    """Three-leg [ket, operator leg 2K = rank, bra] operator holding the blocks in ``op``."""
     n = len(twoj)
     present = np.zeros(n * n, dtype=np.int32)
     pos = {j: i for i, j in enumerate(twoj)}
     for (a, b) in op:
         present[pos[a] + n * pos[b]] = 1
-    t = Tensor(ctx, [(twoj, dims, 1), ([2], [1], 1), (twoj, dims, -1)], locspin=0,
+    t = Tensor(ctx, [(twoj, dims, 1), ([rank], [1], 1), (twoj, dims, -1)], locspin=0,
                present=present)
     flat = np.zeros(t.size)
     for _, tj, dm, off in t.blocks:
@@ -330,9 +331,11 @@ class HeffPlan:
     (coef, Tensor) summed once at plan creation.
     """
 
-    def __init__(self, ctx, psi_layout, hl, hr, terms):
+    def __init__(self, ctx, psi_layout, hl, hr, terms, ranks=None):
+        """ranks: doubled operator-leg rank 2K of each term (default all 2)."""
         self.ctx = ctx
         lc, rc, lops, rops, lcoef, rcoef = [], [], [], [], [], []
+        rk, _ = _ia(list(ranks) if ranks is not None else [2] * max(1, len(terms)))
         for left, right in terms:
             lc.append(len(left))
             rc.append(len(right))
@@ -351,7 +354,8 @@ class HeffPlan:
         RC = (C.c_double * max(1, len(rcoef)))(*rcoef)
         h = _p()
         ctx.check(lib.su2_heff_create(ctx.h, psi_layout.h, hl.h if hl else None,
-                                      hr.h if hr else None, len(terms), p1, LO, LC, p2, RO, RC,
+                                      hr.h if hr else None, len(terms),
+                                      rk.ctypes.data_as(_ip), p1, LO, LC, p2, RO, RC,
                                       C.byref(h)))
         self.h = h
 
@@ -441,7 +445,22 @@ def recoupling(kind, *labels):
     return out.value
 
 
-def _wrap_tensor(ctx, h):
+ENV_KINDS = {"op_transform": 0, "overlap": 1, "pairup": 2, "initial": 3}
+
+
+def env_transform(ctx, kind, left, block_twoj, block_dims, P=None, op=None, coef=1.0):
+    """su2_env_transform: one environment kernel on one block operator (see the
+    header); returns the new device tensor."""
+    a1, p1 = _ia(block_twoj)
+    a2, p2 = _ia(block_dims)
+    h = _p()
+    ctx.check(lib.su2_env_transform(ctx.h, ENV_KINDS[kind], 0 if left else 1, len(block_twoj),
+                                    p1, p2, P.h if P is not None else None,
+                                    op.h if op is not None else None, float(coef), C.byref(h)))
+    return _wrap_tensor(ctx, h, nleg=2 if kind in ("overlap", "pairup") else 3)
+
+
+def _wrap_tensor(ctx, h, nleg=2):
     t = Tensor.__new__(Tensor)
     t.ctx = ctx
     t.h = h
@@ -456,7 +475,7 @@ def _wrap_tensor(ctx, h):
         dm = (C.c_int * 4)()
         off = _ll()
         ctx.check(lib.su2_tensor_block(h, i, C.byref(idx), tj, dm, C.byref(off)))
-        t.blocks.append((idx.value, tuple(tj[:2]), tuple(dm[:2]), off.value))
+        t.blocks.append((idx.value, tuple(tj[:nleg]), tuple(dm[:nleg]), off.value))
     t.legs = None
     return t
 
