@@ -616,7 +616,7 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
   // pairs of 16-column blocks run one wave per workgroup (several per CU);
   // pairs of 32-column blocks run four waves
   const int CB = block_cols, NP = 2 * CB;
-  const int slice_target = CB == 16 ? 1024 : 256;
+  const int slice_rows = CB == 16 ? 170 : 340;
   std::vector<int> L(nb), K(nb);
   std::vector<long long> noff(nb + 1, 0);
   for (int b = 0; b < nb; ++b) {
@@ -687,24 +687,24 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
         rounds[r].push_back(PairJob{b, P * CB, width(P), width(Q) ? Q * CB : 0, width(Q), 1});
       }
   }
-  // Row slices: a block with few pairs in a round spreads each pair over up
-  // to MAX_SLICES workgroups (at least 64 rows each) so that a round fills the
-  // chip.  The slice count of a pair follows from its own block (its pairs in
-  // the round against slice_target workgroups), never from the other blocks
-  // of the batch: the slices' partial Gram matrices are summed in slice
-  // order, so the slicing decides the bits.
+  // Row slices: each pair is spread over up to MAX_SLICES workgroups of
+  // about slice_rows rows each (at least 64), so that a round of a long block
+  // still fills the chip and no workgroup of the round holds far more rows
+  // than another.  The slice count of a pair follows from its own block's
+  // rows alone, never from the other blocks of the batch: the slices'
+  // partial Gram matrices are summed in slice order, so the slicing decides
+  // the bits.  (On the 16x8 D=2000 sweep, 340-row slices -- 4 per pair of the
+  // 1362-row sectors -- took 21 s of Jacobi; 8 slices of 170 rows 26 s.)
   std::vector<int> round_off(rounds.size() + 1, 0), round_pmax(rounds.size(), 1);
   std::vector<PairJob> all;
   size_t gpart_max = 0;
   for (size_t r = 0; r < rounds.size(); ++r) {
     round_off[r] = (int)all.size();
     const int nj = (int)rounds[r].size();
-    std::map<int, int> per_block;
-    for (const PairJob& j : rounds[r]) per_block[j.blk]++;
     int used = 1;
     for (PairJob& j : rounds[r]) {
-      const int pmax = std::min(MAX_SLICES, std::max(1, slice_target / per_block[j.blk]));
-      j.P = std::max(1, std::min(pmax, L[j.blk] / MIN_SLICE_ROWS));
+      const int want = (L[j.blk] + slice_rows - 1) / slice_rows;
+      j.P = std::max(1, std::min(std::min(MAX_SLICES, want), L[j.blk] / MIN_SLICE_ROWS));
       used = std::max(used, j.P);
     }
     round_pmax[r] = used;
