@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "common.hpp"
 #include "context.hpp"
@@ -735,9 +736,7 @@ GroupedGemm::~GroupedGemm() { release(); }
 
 void GroupedGemm::release() {
   if (pool_) {
-    pool_->free(d_probs_);
-    pool_->free(d_slices_);
-    pool_->free(d_reds_);
+    pool_->free(d_probs_);   // one block holds the problem, slice and reduce tables
     pool_->free(d_ws_);
   }
   d_probs_ = nullptr;
@@ -814,6 +813,7 @@ void GroupedGemm::finalize(Context& c, int min_units, int kt_per_unit, int varia
     SU2_CHECK(p.slice_end > p.slice_begin, kErrArgument, "gemm: problem without K slices");
   min_units_ = std::max(1, min_units);
   kt_per_unit_ = std::max(1, kt_per_unit);
+  stream_ = c.stream();
   set_variant(variant < 0 ? kDefaultGemmVariant : variant);
 }
 
@@ -907,22 +907,25 @@ void GroupedGemm::set_variant(int v) {
     grid_ = 8 * longest;
     SU2_CHECK(grid_ < (1ll << 31), kErrCapacity, "gemm: grid of %lld workgroups", grid_);
   }
-  if (!run_probs_.empty()) {
-    d_probs_ = (GemmProblem*)pool_->alloc(run_probs_.size() * sizeof(GemmProblem));
-    SU2_HIP(hipMemcpy(d_probs_, run_probs_.data(), run_probs_.size() * sizeof(GemmProblem),
-                      hipMemcpyHostToDevice));
+  // the three tables in one block, uploaded with one copy on the plan's
+  // stream (ordered before every launch of the plan)
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t bp = al(run_probs_.size() * sizeof(GemmProblem));
+  const size_t bs = al(sl.size() * sizeof(GemmSlice));
+  const size_t br = reds_.size() * sizeof(ReduceProblem);
+  if (bp + bs + br > 0) {
+    std::vector<char> host(bp + bs + br, 0);
+    std::memcpy(host.data(), run_probs_.data(), run_probs_.size() * sizeof(GemmProblem));
+    std::memcpy(host.data() + bp, sl.data(), sl.size() * sizeof(GemmSlice));
+    std::memcpy(host.data() + bp + bs, reds_.data(), br);
+    char* d = (char*)pool_->alloc(host.size());
+    SU2_HIP(hipMemcpyAsync(d, host.data(), host.size(), hipMemcpyHostToDevice, stream_));
+    d_probs_ = run_probs_.empty() ? nullptr : (GemmProblem*)d;
+    d_slices_ = sl.empty() ? nullptr : (GemmSlice*)(d + bp);
+    d_reds_ = reds_.empty() ? nullptr : (ReduceProblem*)(d + bp + bs);
+    if (!d_probs_) d_probs_ = (GemmProblem*)d;   // the block is freed through d_probs_
   }
-  if (!sl.empty()) {
-    d_slices_ = (GemmSlice*)pool_->alloc(sl.size() * sizeof(GemmSlice));
-    SU2_HIP(hipMemcpy(d_slices_, sl.data(), sl.size() * sizeof(GemmSlice),
-                      hipMemcpyHostToDevice));
-  }
-  if (!reds_.empty()) {
-    d_reds_ = (ReduceProblem*)pool_->alloc(reds_.size() * sizeof(ReduceProblem));
-    SU2_HIP(hipMemcpy(d_reds_, reds_.data(), reds_.size() * sizeof(ReduceProblem),
-                      hipMemcpyHostToDevice));
-    d_ws_ = pool_->alloc(ws * elem_size(prec_));
-  }
+  if (!reds_.empty()) d_ws_ = pool_->alloc(ws * elem_size(prec_));
 }
 
 void GroupedGemm::launch(hipStream_t stream) const {

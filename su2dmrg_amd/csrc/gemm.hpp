@@ -128,6 +128,7 @@ class GroupedGemm {
   int min_units_ = 256;
   int kt_per_unit_ = 64;
   std::shared_ptr<DevicePool> pool_;   // device memory of the finalizing context
+  hipStream_t stream_ = nullptr;       // the finalizing context's stream (table uploads)
   int variant_ = -1;
   int layout_ = -1;
   bool uniform_ = true;

@@ -250,7 +250,8 @@ HeffPlan::HeffPlan(Context& c, const Tensor& psi, const Tensor* hl, const Tensor
   instantiate(stage2_, spec2_, bases, sizeof(double));
   stage1_.finalize(c, kMinUnits1, kKtPerUnit1);
   stage2_.finalize(c, kMinUnits2, kKtPerUnit2);
-  SU2_HIP(hipStreamSynchronize(c.stream()));
+  // no wait: the host-side tables above were copied when their uploads were
+  // queued, and every later use of the plan is ordered on the stream
 }
 
 void HeffPlan::instantiate(GroupedGemm& g, const std::vector<GemmSpec>& spec, void* const* bases,

@@ -8,6 +8,7 @@
 #include <memory>
 #include <numeric>
 
+#include "blas1.hpp"
 #include "common.hpp"
 #include "context.hpp"
 
@@ -590,6 +591,96 @@ __global__ __launch_bounds__(256) void col_sign_kernel(const double* __restrict_
   if (threadIdx.x == 0) sign[blockIdx.x] = (si[0] < rows && col[si[0]] < 0.0) ? -1.0 : 1.0;
 }
 
+// ---- batched helpers: one launch for every block of the batch ----
+
+// block of global column gc (columns of all blocks concatenated)
+__device__ __forceinline__ int col_block(const long long* __restrict__ noff, int nb, long long gc) {
+  int lo = 0, hi = nb - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (noff[mid] <= gc) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// squared column norms of every block's current W (one workgroup per column)
+__global__ __launch_bounds__(256) void colnorm_batched_kernel(const BlockDesc* __restrict__ desc,
+                                                              const long long* __restrict__ noff,
+                                                              int nb, double* __restrict__ out) {
+  __shared__ double s[256];
+  const long long gc = blockIdx.x;
+  const int b = col_block(noff, nb, gc);
+  const BlockDesc B = desc[b];
+  const double* col = B.W + (gc - noff[b]) * B.L;
+  double a = 0.0;
+  for (int r = threadIdx.x; r < B.L; r += 256) a += col[r] * col[r];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[gc] = s[0];
+}
+
+// dst[:, c] = sign[c] * src[:, perm[c]] for every job (perm and sign hold
+// the batch's columns, a job's at its block's column offset)
+struct PermJob {
+  const double* src;
+  double* dst;
+  int rows, cols;
+  long long elem_begin, col_off;
+};
+
+__global__ void permute_batched_kernel(const PermJob* __restrict__ jobs, int nj, long long total,
+                                       const int* __restrict__ perm,
+                                       const double* __restrict__ sign) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  int lo = 0, hi = nj - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].elem_begin <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  const PermJob J = jobs[lo];
+  const long long i = e - J.elem_begin;
+  const int r = (int)(i % J.rows), c = (int)(i / J.rows);
+  J.dst[i] = sign[J.col_off + c] * J.src[r + (long long)perm[J.col_off + c] * J.rows];
+}
+
+// col_sign_kernel for the V of every block (desc[b].V, k x k)
+__global__ __launch_bounds__(256) void col_sign_batched_kernel(const BlockDesc* __restrict__ desc,
+                                                               const long long* __restrict__ noff,
+                                                               int nb, double* __restrict__ sign) {
+  __shared__ double sv[256];
+  __shared__ int si[256];
+  const long long gc = blockIdx.x;
+  const int b = col_block(noff, nb, gc);
+  const int rows = desc[b].k;
+  const double* col = desc[b].V + (gc - noff[b]) * rows;
+  double best = 0.0;
+  for (int r = threadIdx.x; r < rows; r += 256) best = fmax(best, fabs(col[r]));
+  sv[threadIdx.x] = best;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sv[threadIdx.x] = fmax(sv[threadIdx.x], sv[threadIdx.x + w]);
+    __syncthreads();
+  }
+  const double half = 0.5 * sv[0];
+  int first = 0x7fffffff;
+  for (int r = threadIdx.x; r < rows; r += 256)
+    if (fabs(col[r]) >= half && r < first) first = r;
+  si[threadIdx.x] = first;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) si[threadIdx.x] = min(si[threadIdx.x], si[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sign[gc] = (si[0] < rows && col[si[0]] < 0.0) ? -1.0 : 1.0;
+}
+
 unsigned nblk(long long n, int t) { return (unsigned)std::max<long long>(1, (n + t - 1) / t); }
 
 // device memory from the context's cache, returned on scope exit
@@ -616,7 +707,7 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
   // pairs of 16-column blocks run one wave per workgroup (several per CU);
   // pairs of 32-column blocks run four waves
   const int CB = block_cols, NP = 2 * CB;
-  const int slice_rows = CB == 16 ? 170 : 340;
+  const int slice_target = CB == 16 ? 1024 : 256;
   std::vector<int> L(nb), K(nb);
   std::vector<long long> noff(nb + 1, 0);
   for (int b = 0; b < nb; ++b) {
@@ -625,6 +716,8 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
     K[b] = std::min(blocks[b].m, blocks[b].n);
     noff[b + 1] = noff[b] + K[b];
   }
+  const long long ncol = noff[nb];
+  SU2_CHECK(ncol < (1LL << 31), kErrCapacity, "jacobi: %lld columns", ncol);
   // second buffers: each sweep starts by moving the columns into descending
   // norm order (de Rijk's ordering; it cuts the sweep count severalfold)
   std::vector<std::unique_ptr<DevMem>> W2(nb), V2(nb);
@@ -643,28 +736,55 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
                        K[b]);
   }
   SU2_HIP(hipGetLastError());
-  DevMem dn(c, sizeof(double) * noff[nb]), dperm(c, sizeof(int) * noff[nb]),
-      done(c, sizeof(double) * noff[nb]);
-  std::vector<double> hn(noff[nb]);
-  std::vector<int> hp(noff[nb]);
+  DevMem dn(c, sizeof(double) * ncol), dperm(c, sizeof(int) * ncol),
+      done(c, sizeof(double) * ncol), dsign(c, sizeof(double) * ncol),
+      dnoff(c, sizeof(long long) * (nb + 1)), dpj(c, sizeof(PermJob) * 2 * nb);
+  std::vector<double> hn(ncol);
+  std::vector<int> hp(ncol);
   {
-    std::vector<double> ones(noff[nb], 1.0);
-    SU2_HIP(hipMemcpyAsync(done.p, ones.data(), sizeof(double) * noff[nb], hipMemcpyHostToDevice,
-                           st));
+    std::vector<double> ones(ncol, 1.0);
+    SU2_HIP(hipMemcpyAsync(done.p, ones.data(), sizeof(double) * ncol, hipMemcpyHostToDevice, st));
+    SU2_HIP(hipMemcpyAsync(dnoff.p, noff.data(), sizeof(long long) * (nb + 1),
+                           hipMemcpyHostToDevice, st));
   }
+  std::vector<BlockDesc> desc(nb);
+  DevMem ddesc(c, sizeof(BlockDesc) * nb);
+  auto upload_desc = [&]() {
+    for (int b = 0; b < nb; ++b) {
+      desc[b].W = Wc[b];
+      desc[b].V = Vc[b];
+    }
+    SU2_HIP(hipMemcpyAsync(ddesc.p, desc.data(), sizeof(BlockDesc) * nb, hipMemcpyHostToDevice,
+                           st));
+  };
+  // squared column norms of the current W of every block, to the host
   auto norms = [&]() {
-    for (int b = 0; b < nb; ++b)
-      hipLaunchKernelGGL(colnorm_kernel, dim3(K[b]), dim3(256), 0, st, Wc[b], L[b],
-                         (double*)dn.p + noff[b]);
-    SU2_HIP(hipMemcpyAsync(hn.data(), dn.p, sizeof(double) * noff[nb], hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(colnorm_batched_kernel, dim3((unsigned)ncol), dim3(256), 0, st,
+                       (const BlockDesc*)ddesc.p, (const long long*)dnoff.p, nb, (double*)dn.p);
+    SU2_HIP(hipMemcpyAsync(hn.data(), dn.p, sizeof(double) * ncol, hipMemcpyDeviceToHost, st));
     SU2_HIP(hipStreamSynchronize(st));
   };
-  std::vector<BlockDesc> desc(nb);
+  // src -> dst column moves of W and V (or the given jobs) in one launch
+  auto permute = [&](const std::vector<PermJob>& pj0, const double* sign) {
+    std::vector<PermJob> pj = pj0;
+    long long tot = 0;
+    for (auto& j : pj) {
+      j.elem_begin = tot;
+      tot += (long long)j.rows * j.cols;
+    }
+    if (tot == 0) return;
+    SU2_HIP(hipMemcpyAsync(dpj.p, pj.data(), sizeof(PermJob) * pj.size(), hipMemcpyHostToDevice,
+                           st));
+    hipLaunchKernelGGL(permute_batched_kernel, dim3(nblk(tot, 256)), dim3(256), 0, st,
+                       (const PermJob*)dpj.p, (int)pj.size(), tot, (const int*)dperm.p, sign);
+  };
+  for (int b = 0; b < nb; ++b) desc[b] = BlockDesc{Wc[b], Vc[b], L[b], K[b], 0.0};
+  upload_desc();
   norms();
   for (int b = 0; b < nb; ++b) {
     double g = 0;
     for (int i = 0; i < K[b]; ++i) g += hn[noff[b] + i];
-    desc[b] = BlockDesc{nullptr, nullptr, L[b], K[b], std::sqrt(g)};
+    desc[b].normF = std::sqrt(g);
   }
   // round-robin schedule of column-block pairs, merged across blocks by round
   std::vector<std::vector<PairJob>> rounds;
@@ -687,24 +807,26 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
         rounds[r].push_back(PairJob{b, P * CB, width(P), width(Q) ? Q * CB : 0, width(Q), 1});
       }
   }
-  // Row slices: each pair is spread over up to MAX_SLICES workgroups of
-  // about slice_rows rows each (at least 64), so that a round of a long block
-  // still fills the chip and no workgroup of the round holds far more rows
-  // than another.  The slice count of a pair follows from its own block's
-  // rows alone, never from the other blocks of the batch: the slices'
+  // Row slices: a group (the sectors of one wavefunction) with few pairs in
+  // a round spreads each pair over up to MAX_SLICES workgroups (at least 64
+  // rows each), about slice_target workgroups per group and round, so that
+  // the round fills the chip.  The slice count of a pair follows from its own
+  // group alone, never from the other groups of the batch: the slices'
   // partial Gram matrices are summed in slice order, so the slicing decides
-  // the bits.  (On the 16x8 D=2000 sweep, 340-row slices -- 4 per pair of the
-  // 1362-row sectors -- took 21 s of Jacobi; 8 slices of 170 rows 26 s.)
+  // the bits, and a wavefunction gets the same bits alone and in a batch.
   std::vector<int> round_off(rounds.size() + 1, 0), round_pmax(rounds.size(), 1);
   std::vector<PairJob> all;
   size_t gpart_max = 0;
   for (size_t r = 0; r < rounds.size(); ++r) {
     round_off[r] = (int)all.size();
     const int nj = (int)rounds[r].size();
+    std::map<int, int> per_group;
+    for (const PairJob& j : rounds[r]) per_group[blocks[j.blk].group]++;
     int used = 1;
     for (PairJob& j : rounds[r]) {
-      const int want = (L[j.blk] + slice_rows - 1) / slice_rows;
-      j.P = std::max(1, std::min(std::min(MAX_SLICES, want), L[j.blk] / MIN_SLICE_ROWS));
+      const int pmax =
+          std::min(MAX_SLICES, std::max(1, slice_target / per_group[blocks[j.blk].group]));
+      j.P = std::max(1, std::min(pmax, L[j.blk] / MIN_SLICE_ROWS));
       used = std::max(used, j.P);
     }
     round_pmax[r] = used;
@@ -713,7 +835,7 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
   }
   round_off[rounds.size()] = (int)all.size();
   DevMem dgpart(c, sizeof(double) * NP * NP * std::max<size_t>(1, gpart_max));
-  DevMem ddesc(c, sizeof(BlockDesc) * nb), djobs(c, sizeof(PairJob) * std::max<size_t>(1, all.size())),
+  DevMem djobs(c, sizeof(PairJob) * std::max<size_t>(1, all.size())),
       doff(c, sizeof(unsigned long long) * nb);
   if (!all.empty())
     SU2_HIP(hipMemcpyAsync(djobs.p, all.data(), sizeof(PairJob) * all.size(),
@@ -732,21 +854,20 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
       std::iota(p, p + K[b], 0);
       std::stable_sort(p, p + K[b], [&](int x, int y) { return w[x] > w[y]; });
     }
-    SU2_HIP(hipMemcpyAsync(dperm.p, hp.data(), sizeof(int) * noff[nb], hipMemcpyHostToDevice, st));
+    SU2_HIP(hipMemcpyAsync(dperm.p, hp.data(), sizeof(int) * ncol, hipMemcpyHostToDevice, st));
+    {
+      std::vector<PermJob> pj;
+      for (int b = 0; b < nb; ++b) {
+        pj.push_back(PermJob{Wc[b], Wo[b], L[b], K[b], 0, noff[b]});
+        pj.push_back(PermJob{Vc[b], Vo[b], K[b], K[b], 0, noff[b]});
+      }
+      permute(pj, (const double*)done.p);
+    }
     for (int b = 0; b < nb; ++b) {
-      const int* pb = (const int*)dperm.p + noff[b];
-      const double* one = (const double*)done.p + noff[b];
-      hipLaunchKernelGGL(permute_cols_kernel, dim3(nblk((long long)L[b] * K[b], 256)), dim3(256),
-                         0, st, Wc[b], Wo[b], pb, one, L[b], K[b]);
-      hipLaunchKernelGGL(permute_cols_kernel, dim3(nblk((long long)K[b] * K[b], 256)), dim3(256),
-                         0, st, Vc[b], Vo[b], pb, one, K[b], K[b]);
       std::swap(Wc[b], Wo[b]);
       std::swap(Vc[b], Vo[b]);
-      desc[b].W = Wc[b];
-      desc[b].V = Vc[b];
     }
-    SU2_HIP(hipMemcpyAsync(ddesc.p, desc.data(), sizeof(BlockDesc) * nb, hipMemcpyHostToDevice,
-                           st));
+    upload_desc();
     hipLaunchKernelGGL(fill_u64_kernel, dim3(nblk(nb, 256)), dim3(256), 0, st,
                        (unsigned long long*)doff.p, nb);
     for (size_t r = 0; r < rounds.size(); ++r) {
@@ -789,43 +910,67 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
             max_sweeps, worst);
 
   // singular values in ascending order, sign convention on V (W follows),
-  // results into the caller's buffers
+  // results into the caller's buffers -- for every block in a few launches
   norms();
+  std::vector<int> ident(ncol);
   for (int b = 0; b < nb; ++b) {
     JacobiBlock& J = blocks[b];
-    const int k = K[b], l = L[b];
+    const int k = K[b];
     const double* w2 = hn.data() + noff[b];
-    std::vector<int> perm(k);
-    std::iota(perm.begin(), perm.end(), 0);
-    std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return w2[x] < w2[y]; });
+    int* perm = hp.data() + noff[b];
+    std::iota(perm, perm + k, 0);
+    std::iota(ident.data() + noff[b], ident.data() + noff[b] + k, 0);
+    std::stable_sort(perm, perm + k, [&](int x, int y) { return w2[x] < w2[y]; });
     J.s.resize(k);
     for (int i = 0; i < k; ++i) J.s[i] = std::sqrt(w2[perm[i]]);
-    DevMem dpb(c, sizeof(int) * k), dsign(c, sizeof(double) * k), did(c, sizeof(int) * k);
-    std::vector<int> ident(k);
-    std::iota(ident.begin(), ident.end(), 0);
-    SU2_HIP(hipMemcpyAsync(dpb.p, perm.data(), sizeof(int) * k, hipMemcpyHostToDevice, st));
-    SU2_HIP(hipMemcpyAsync(did.p, ident.data(), sizeof(int) * k, hipMemcpyHostToDevice, st));
-    const double* one = (const double*)done.p + noff[b];
-    // V: sorted copy in the spare buffer, then the signs, then into J.V
-    hipLaunchKernelGGL(permute_cols_kernel, dim3(nblk((long long)k * k, 256)), dim3(256), 0, st,
-                       Vc[b], Vo[b], (const int*)dpb.p, one, k, k);
-    hipLaunchKernelGGL(col_sign_kernel, dim3(k), dim3(256), 0, st, (const double*)Vo[b], k,
-                       (double*)dsign.p);
-    hipLaunchKernelGGL(permute_cols_kernel, dim3(nblk((long long)k * k, 256)), dim3(256), 0, st,
-                       Vo[b], J.V == Vo[b] ? Vc[b] : J.V, (const int*)did.p,
-                       (const double*)dsign.p, k, k);
-    // W: same permutation and signs
+  }
+  DevMem dident(c, sizeof(int) * ncol);
+  SU2_HIP(hipMemcpyAsync(dperm.p, hp.data(), sizeof(int) * ncol, hipMemcpyHostToDevice, st));
+  SU2_HIP(hipMemcpyAsync(dident.p, ident.data(), sizeof(int) * ncol, hipMemcpyHostToDevice, st));
+  // V: sorted copy in the spare buffer, its signs
+  {
+    std::vector<PermJob> pj;
+    for (int b = 0; b < nb; ++b) pj.push_back(PermJob{Vc[b], Vo[b], K[b], K[b], 0, noff[b]});
+    permute(pj, (const double*)done.p);
+  }
+  for (int b = 0; b < nb; ++b) {
+    desc[b].W = Wc[b];
+    desc[b].V = Vo[b];
+  }
+  SU2_HIP(hipMemcpyAsync(ddesc.p, desc.data(), sizeof(BlockDesc) * nb, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(col_sign_batched_kernel, dim3((unsigned)ncol), dim3(256), 0, st,
+                     (const BlockDesc*)ddesc.p, (const long long*)dnoff.p, nb, (double*)dsign.p);
+  // V with its signs into J.V (or the other buffer, when J.V is the spare),
+  // W with the same order and signs into J.W (or the spare)
+  std::vector<PermJob> vj, wj;
+  std::vector<VecMove> moves;
+  for (int b = 0; b < nb; ++b) {
+    JacobiBlock& J = blocks[b];
+    double* vdst = J.V == Vo[b] ? Vc[b] : J.V;
+    vj.push_back(PermJob{Vo[b], vdst, K[b], K[b], 0, noff[b]});
     double* wdst = J.W == Wc[b] ? Wo[b] : J.W;
-    hipLaunchKernelGGL(permute_cols_kernel, dim3(nblk((long long)l * k, 256)), dim3(256), 0, st,
-                       Wc[b], wdst, (const int*)dpb.p, (const double*)dsign.p, l, k);
-    if (wdst != J.W)
-      SU2_HIP(hipMemcpyAsync(J.W, wdst, sizeof(double) * (size_t)l * k, hipMemcpyDeviceToDevice,
-                             st));
-    if (J.V == Vo[b])
-      SU2_HIP(hipMemcpyAsync(J.V, Vc[b], sizeof(double) * (size_t)k * k, hipMemcpyDeviceToDevice,
-                             st));
+    wj.push_back(PermJob{Wc[b], wdst, L[b], K[b], 0, noff[b]});
+    if (wdst != J.W) moves.push_back(VecMove{wdst, J.W, (long long)L[b] * K[b], kMoveF64, 0});
+    if (vdst != J.V) moves.push_back(VecMove{vdst, J.V, (long long)K[b] * K[b], kMoveF64, 0});
+  }
+  {
+    // the V moves read the identity permutation, the W moves the sort order
+    std::swap(dperm.p, dident.p);
+    permute(vj, (const double*)dsign.p);
+    std::swap(dperm.p, dident.p);
+    SU2_HIP(hipStreamSynchronize(st));   // the job table is reused below
+    permute(wj, (const double*)dsign.p);
+  }
+  if (!moves.empty()) {
+    long long mx = 0;
+    for (auto& m : moves) mx = std::max(mx, m.n);
+    DevMem dm(c, sizeof(VecMove) * moves.size());
+    SU2_HIP(hipMemcpyAsync(dm.p, moves.data(), sizeof(VecMove) * moves.size(),
+                           hipMemcpyHostToDevice, st));
+    dev_move_batch(c, (const VecMove*)dm.p, (int)moves.size(), mx);
     SU2_HIP(hipStreamSynchronize(st));
   }
+  SU2_HIP(hipStreamSynchronize(st));
   SU2_HIP(hipGetLastError());
   return sweeps_done;
 }

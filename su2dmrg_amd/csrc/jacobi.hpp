@@ -40,6 +40,9 @@ struct JacobiBlock {
                      //      ascending singular value (device, caller-allocated)
   double* V;         // out: k x k, ld k, matching columns (device, caller-allocated)
   std::vector<double> s;   // out: k singular values, ascending
+  int group = 0;           // blocks of one group (one wavefunction) share the
+                           // row slicing of their pairs; groups never affect
+                           // each other's bits
 };
 
 // Decomposes every block and returns the number of sweeps it took; throws

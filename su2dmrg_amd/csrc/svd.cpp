@@ -22,17 +22,16 @@ struct Sec {
   std::vector<double> s;   // k values, ascending, zeros for cut
 };
 
-// singular values, keep rule, global cut and output tensors of one
-// wavefunction whose sectors' triplets are in secs (Jacobi done)
-SvdResult finish_svd(Context& c, const Tensor& psi, std::vector<Sec>& secs,
-                     std::vector<std::vector<double>>& wv, double p1, double p2, int max_dcut) {
+// Singular values, keep rule and column factors of one wavefunction's
+// sectors (Jacobi done): the factors s^(p-1) / s^p of every sector go into
+// `fac` (host), the sector's offset into foff.
+void svd_values(std::vector<Sec>& secs, std::vector<std::vector<double>>& wv, double p1,
+                double p2, int max_dcut, std::vector<double>& fac, std::vector<long long>& foff) {
   const double tol = 1e-100;
   // singular values and column scaling (tensor.cpp:1533-1568, 1620-1655)
-  std::vector<double> wsort;
-  std::vector<int> isort;
-  double* dfac = nullptr;
   for (size_t i = 0; i < secs.size(); ++i) {
     Sec& s = secs[i];
+    foff.push_back((long long)fac.size());
     if (s.k == 0) continue;
     std::vector<double>& w = wv[i];
     const int k = s.k;
@@ -63,17 +62,19 @@ SvdResult finish_svd(Context& c, const Tensor& psi, std::vector<Sec>& secs,
       fe[t] = std::pow(s.s[t], pe);
       if (s.s[t] > noise || pl >= 0) fl[t] = std::pow(s.s[t], pl);
     }
-    dfac = c.reduce_scratch(2 * k);
-    std::vector<double> both(fl);
-    both.insert(both.end(), fe.begin(), fe.end());
-    SU2_HIP(hipMemcpyAsync(dfac, both.data(), 2 * k * sizeof(double), hipMemcpyHostToDevice,
-                           c.stream()));
-    const int longrows = s.m >= s.n ? s.m : s.n;
-    dev_scale_columns(c, s.U.p, longrows, k, dfac);
-    dev_scale_columns(c, s.V.p, k, k, dfac + k);
-    SU2_HIP(hipStreamSynchronize(c.stream()));
-    for (int t = 0; t < dc; ++t) {
-      wsort.push_back(s.s[k - dc + t]);
+    fac.insert(fac.end(), fl.begin(), fl.end());
+    fac.insert(fac.end(), fe.begin(), fe.end());
+  }
+}
+
+// Global cut and output tensors of one wavefunction (columns already scaled).
+SvdResult svd_cut(Context& c, const Tensor& psi, std::vector<Sec>& secs, int max_dcut) {
+  std::vector<double> wsort;
+  std::vector<int> isort;
+  for (size_t i = 0; i < secs.size(); ++i) {
+    const Sec& s = secs[i];
+    for (int t = 0; t < s.dc; ++t) {
+      wsort.push_back(s.s[s.k - s.dc + t]);
       isort.push_back((int)i);
     }
   }
@@ -128,7 +129,6 @@ SvdResult finish_svd(Context& c, const Tensor& psi, std::vector<Sec>& secs,
     std::vector<double> kv(s.s.begin() + (k - d), s.s.end());
     res.kept.push_back(kv);
   }
-  SU2_HIP(hipStreamSynchronize(c.stream()));
   return res;
 }
 
@@ -181,6 +181,7 @@ std::vector<SvdResult> truncated_svd_batch(Context& c, const std::vector<SvdJob>
         b.n = s.n;
         b.W = s.U.p;
         b.V = s.V.p;
+        b.group = (int)q;
         jb.push_back(b);
         idx.push_back({q, i});
       }
@@ -194,13 +195,31 @@ std::vector<SvdResult> truncated_svd_batch(Context& c, const std::vector<SvdJob>
       for (size_t q = 0; q < w.size(); ++q) w[q] = jb[t].s[q] * jb[t].s[q];
     }
   }
+  // column factors of every sector of every wavefunction: one upload, the
+  // scalings queued, no wait until the outputs are in place
+  std::vector<double> fac;
+  std::vector<std::vector<long long>> foff(jobs.size());
+  for (size_t q = 0; q < jobs.size(); ++q)
+    svd_values(secs[q], wv[q], jobs[q].p1, jobs[q].p2, jobs[q].max_dcut, fac, foff[q]);
+  DevBuf dfac(c, (long long)fac.size());
+  if (!fac.empty())
+    SU2_HIP(hipMemcpyAsync(dfac.p, fac.data(), fac.size() * sizeof(double), hipMemcpyHostToDevice,
+                           c.stream()));
+  for (size_t q = 0; q < jobs.size(); ++q)
+    for (size_t i = 0; i < secs[q].size(); ++i) {
+      Sec& s = secs[q][i];
+      if (s.k == 0) continue;
+      const int longrows = s.m >= s.n ? s.m : s.n;
+      dev_scale_columns(c, s.U.p, longrows, s.k, dfac.p + foff[q][i]);
+      dev_scale_columns(c, s.V.p, s.k, s.k, dfac.p + foff[q][i] + s.k);
+    }
   std::vector<SvdResult> out;
   for (size_t q = 0; q < jobs.size(); ++q) {
-    out.push_back(finish_svd(c, *jobs[q].psi, secs[q], wv[q], jobs[q].p1, jobs[q].p2,
-                             jobs[q].max_dcut));
+    out.push_back(svd_cut(c, *jobs[q].psi, secs[q], jobs[q].max_dcut));
     out.back().jacobi_sweeps = jacobi_sweeps;
     out.back().t_jacobi = t_jacobi / jobs.size();
   }
+  SU2_HIP(hipStreamSynchronize(c.stream()));
   return out;
 }
 
