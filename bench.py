@@ -233,17 +233,16 @@ def scan_leg(L, ctx, njobs, nsweep, dvals):
             "batch_stats": {k: round(v, 3) for k, v in st.items()}}
 
 
-def serve_leg(L, ctx, njobs, rate_per_s, seed=3):
+def serve_leg(L, ctx, njobs, rate_per_s, lattices, dvals, seed=3):
     import numpy as np
     rng = np.random.default_rng(seed)
-    lattices = [(6, 4), (8, 4), (8, 6), (12, 6), (16, 8)]
     jobs = []
     for k in range(njobs):
         lx, ly = lattices[int(rng.integers(len(lattices)))]
         j2 = rng.uniform(0.3, 0.7)
         q = rng.uniform(0.0, 2.0)
         s = j2 + q
-        D = int(rng.choice((64, 100, 150)))
+        D = int(rng.choice(dvals))
         exci = int(rng.choice((1, 2))) if k % 2 else 0
         jobs.append(L.Job(lx, ly, jcoup=1.0 / s, qcoup=q / s, max_dcut=D, exci=exci, nsweep=1))
     arrivals = np.cumsum(rng.exponential(1.0 / rate_per_s, njobs))
@@ -267,7 +266,7 @@ def serve_leg(L, ctx, njobs, rate_per_s, seed=3):
     lat1 = np.array([r["latency"] for r in res1])
     same = all(a["energy"] == b["energy"] for a, b in zip(res, res1))
     return {"jobs": njobs, "poisson_rate_per_s": rate_per_s,
-            "lattices": [f"{a}x{b}" for a, b in lattices], "D": [64, 100, 150],
+            "lattices": [f"{a}x{b}" for a, b in lattices], "D": list(dvals),
             "excited_share": 0.5, "slots": 16, "budget_bytes": budget,
             "wall_s": round(st["wall"], 3), "max_concurrent": int(st["max_concurrent"]),
             "jobs_per_hour": round(3600 * njobs / st["wall"], 1),
@@ -324,11 +323,13 @@ def main():
     ap.add_argument("--sweep-d", type=int, default=2000)
     ap.add_argument("--sweeps", type=int, default=3,
                     help="sweeps of each config-3 run (the mixed run: ramp, then two FP64)")
-    ap.add_argument("--scan-jobs", type=int, default=16)
+    ap.add_argument("--scan-jobs", type=int, default=32)
     ap.add_argument("--scan-sweeps", type=int, default=1)
-    ap.add_argument("--scan-d", default="100,200", help="bond dimensions drawn by the scan jobs")
-    ap.add_argument("--serve-jobs", type=int, default=12)
-    ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--scan-d", default="300,500", help="bond dimensions drawn by the scan jobs")
+    ap.add_argument("--serve-jobs", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=8.0)
+    ap.add_argument("--serve-lattices", default="6x4,8x4,8x6,12x6")
+    ap.add_argument("--serve-d", default="64,100,150")
     ap.add_argument("--cpu-sweep", default="16x8:64")
     ap.add_argument("--svd-block", type=int, default=32, choices=[16, 32])
     ap.add_argument("--svd-inner", type=int, default=1, choices=[1, 2, 3])
@@ -416,7 +417,9 @@ def main():
     serve = None
     if "serve" in legs:
         log(f"serve: {args.serve_jobs} jobs at {args.serve_rate}/s")
-        serve = serve_leg(L, ctx, args.serve_jobs, args.serve_rate)
+        serve = serve_leg(L, ctx, args.serve_jobs, args.serve_rate,
+                          [tuple(int(y) for y in x.split("x")) for x in args.serve_lattices.split(",")],
+                          [int(x) for x in args.serve_d.split(",")])
         log(f"serve: {serve['jobs_per_hour']} jobs/h, p50 {serve['p50_latency_s']} s, p99 "
             f"{serve['p99_latency_s']} s; sequential {serve['sequential']['jobs_per_hour']} jobs/h")
 

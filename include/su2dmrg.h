@@ -86,6 +86,9 @@ int su2_ctx_set_svd_tuning(su2_ctx* ctx, int block_cols, int inner_sweeps);
 int su2_tensor_create(su2_ctx* ctx, int nleg, const int* nsec, const int* twoj, const int* dims,
                       const int* dirs, int locspin, const int* present, su2_tensor** out);
 int su2_tensor_destroy(su2_ctx* ctx, su2_tensor* t);
+/* A new zeroed tensor with the legs, locspin and present blocks of src (the
+ * structure half of the reference's copy constructor, tensor_su2.cpp:48-58). */
+int su2_tensor_like(su2_ctx* ctx, const su2_tensor* src, su2_tensor** out);
 /* Number of reduced doubles and of present blocks. */
 int su2_tensor_size(const su2_tensor* t, long long* ndouble, int* nblock);
 /* Block i (0..nblock-1, block-index order): block index, labels, dims, offset. */
@@ -311,6 +314,15 @@ int su2_dmrg_history(const su2_dmrg* d, int* kept, double* discarded, int* matve
  * than they save).  A mixed run schedules 32 for the early sweeps and 64 for
  * the final ones. */
 int su2_dmrg_set_matvec_precision(su2_ctx* ctx, su2_dmrg* d, int bits, double min_flops);
+/* The two-site H_eff of the centre bond (il = ns/2 - 1, ir = ns/2) of the
+ * current state, built from the run's environments as every solve builds it
+ * (dmrg_su2.cpp:543-664, without excited-state penalties), and the state's
+ * wavefunction in that basis (A_j V_j^T per sector, the factors the last
+ * solve left, dmrg_su2_omp.cpp:135-150).  Call after su2_dmrg_warmup or
+ * su2_dmrg_sweep; destroy both with su2_heff_destroy / su2_tensor_destroy.
+ * For measurements on the real state (energy, variance, hermiticity) and
+ * timing the matvec on the run's own sector dimensions. */
+int su2_dmrg_centre(su2_ctx* ctx, su2_dmrg* d, su2_heff** plan, su2_tensor** psi);
 /* H_eff launches of each solve replayed from a HIP graph (on = 1, default)
  * or launched directly (0); the results are the same. */
 int su2_dmrg_set_graph(su2_ctx* ctx, su2_dmrg* d, int on);
@@ -376,12 +388,15 @@ int su2_server_stats(const su2_server* s, double* out, int cap);
  * levels, which hold their lower states). */
 int su2_job_footprint(const su2_job* job, double* bytes);
 
-/* Timing counters: out[0..15] = environment s, Lanczos s, SVD s, projection
+/* Timing counters: out[0..16] = environment s, Lanczos s, SVD s, projection
  * s, start-vector s, device matvec s, algorithmic H_eff FLOPs, matvecs, steps,
  * last centre bond dimension, s inside the one-sided Jacobi SVD, its sweeps
  * summed over solves, FP32 matvecs, their device s, their algorithmic FLOPs,
- * the largest |Ritz value - FP64 Rayleigh quotient| of a mixed solve.  cap
- * bounds what is written. */
+ * the largest |Ritz value - FP64 Rayleigh quotient| of a mixed solve, SVDs
+ * accepted at the Jacobi sweep ceiling (60 sweeps; columns orthogonal to
+ * 1e-6 rather than 1e-13, singular values then good to ~1e-12 relative;
+ * beyond 1e-6 the step fails with SU2_ERR_NUMERIC).  cap bounds what is
+ * written. */
 int su2_dmrg_stats(const su2_dmrg* d, double* out, int cap);
 
 #ifdef __cplusplus

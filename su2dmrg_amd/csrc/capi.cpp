@@ -34,9 +34,9 @@ struct su2_dmrg {
   explicit su2_dmrg(A&&... a) : d(std::forward<A>(a)...) {}
 };
 struct su2_heff {
-  su2::HeffPlan p;
-  template <class... A>
-  explicit su2_heff(A&&... a) : p(std::forward<A>(a)...) {}
+  std::unique_ptr<su2::HeffPlan> own;
+  su2::HeffPlan& p;
+  explicit su2_heff(su2::HeffPlan* q) : own(q), p(*q) {}
 };
 struct su2_batch {
   su2::BatchDriver b;
@@ -165,6 +165,17 @@ int su2_tensor_create(su2_ctx* ctx, int nleg, const int* nsec, const int* twoj, 
       for (long long i = 0; i < nten; ++i) pres[i] = present[i] != 0;
     }
     *out = new su2_tensor(new su2::Tensor(&ctx->c, legs, locspin, present ? &pres : nullptr));
+  });
+}
+
+int su2_tensor_like(su2_ctx* ctx, const su2_tensor* src, su2_tensor** out) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(src, "src");
+    need(out, "out");
+    std::vector<char> present(src->t.nten(), 0);
+    for (auto& b : src->t.blocks()) present[b.index] = 1;
+    *out = new su2_tensor(new su2::Tensor(&ctx->c, src->t.legs(), src->t.locspin(), &present));
   });
 }
 
@@ -341,8 +352,8 @@ int su2_heff_create(su2_ctx* ctx, const su2_tensor* psi_layout, const su2_tensor
         terms[t].right.push_back({right_coef[ri], &right_ops[ri]->t});
       }
     }
-    *out = new su2_heff(ctx->c, psi_layout->t, hl ? &hl->t : nullptr, hr ? &hr->t : nullptr,
-                        terms);
+    *out = new su2_heff(new su2::HeffPlan(ctx->c, psi_layout->t, hl ? &hl->t : nullptr,
+                                          hr ? &hr->t : nullptr, terms));
   });
 }
 
@@ -743,6 +754,18 @@ int su2_batch_stats(const su2_batch* b, double* out, int cap) {
   });
 }
 
+int su2_dmrg_centre(su2_ctx* ctx, su2_dmrg* d, su2_heff** plan, su2_tensor** psi) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    need(d, "dmrg");
+    need(plan, "plan");
+    need(psi, "psi");
+    auto r = d->d.centre();
+    *psi = new su2_tensor(r.second.release());
+    *plan = new su2_heff(r.first.release());
+  });
+}
+
 int su2_dmrg_set_graph(su2_ctx* ctx, su2_dmrg* d, int on) {
   return guard(ctx, [&] {
     need(d, "dmrg");
@@ -839,13 +862,13 @@ int su2_dmrg_stats(const su2_dmrg* d, double* out, int cap) {
     need(d, "dmrg");
     need(out, "out");
     const su2::DmrgStats& s = d->d.stats();
-    const double v[16] = {s.t_env,      s.t_lanczos,         s.t_svd,
+    const double v[17] = {s.t_env,      s.t_lanczos,         s.t_svd,
                           s.t_project,  s.t_start,           s.t_matvec,
                           s.heff_flops, (double)s.matvecs,   (double)s.steps,
                           (double)d->d.last_bond_dim(), s.t_jacobi,
                           (double)s.jacobi_sweeps, (double)s.matvecs32, s.t_matvec32,
-                          s.heff_flops32, s.max_fp64_correction};
-    for (int i = 0; i < 16 && i < cap; ++i) out[i] = v[i];
+                          s.heff_flops32, s.max_fp64_correction, (double)s.svd_degraded};
+    for (int i = 0; i < 17 && i < cap; ++i) out[i] = v[i];
   });
 }
 

@@ -575,6 +575,7 @@ Dmrg::StepOut Dmrg::finish(Solve& S, const LanczosResult& lr, SvdResult&& sv) {
   last_bond_ = (int)sv.bond.total();
   stats_.t_jacobi += sv.t_jacobi;
   stats_.jacobi_sweeps += sv.jacobi_sweeps;
+  stats_.svd_degraded += sv.jacobi_worst > 0 ? 1 : 0;
   if (verbose_) {
     std::fprintf(stderr, "step %lld (%d,%d) E=%.12f kept=%d discarded=%.3e matvecs=%d%s %.3fs\n",
                  (long long)energies_.size() - 1, S.il, S.ir, lr.energy, last_bond_,
@@ -725,6 +726,41 @@ void Dmrg::ensure_right(int i) {
   rb_[i] = E.fused;
   V_[i] = identity_projector(c_, E.fused);
   overlap_right(i);
+}
+
+// The two-site H_eff and wavefunction at the centre bond of the current state
+// (after warmup() or sweep(), whose last solve is there): psi_j = A_j V_j^T
+// per sector, A = U[ns/2 - 1] (carries the singular values), V = V[ns/2].
+std::pair<std::unique_ptr<HeffPlan>, std::unique_ptr<Tensor>> Dmrg::centre() {
+  Context& c = c_;
+  c.make_current();
+  const int il = lat_.ns / 2 - 1, ir = lat_.ns / 2;
+  SU2_CHECK(U_.count(il) && V_.count(ir) && L_.count(il - 1) && R_.count(ir + 1) &&
+                cur_.phase == kPhaseNone,
+            kErrArgument, "centre: no state at the centre bond (run warmup or a sweep first)");
+  Fused FL = fused_left(il), FR = fused_right(ir);
+  std::unique_ptr<Env> El = enlarged_left(il, FL), Er = enlarged_right(ir, FR);
+  std::unique_ptr<Tensor> psi(
+      new Tensor(&c, {with_dir(FL.fused, 1), with_dir(FR.fused, 1)}, 0, nullptr));
+  const Tensor& A = *U_.at(il);
+  const Tensor& V = *V_.at(ir);
+  GemmBatch g;
+  for (auto& b : psi->blocks()) {
+    const int j = b.twoj[0];
+    const int ia = A.find2(j, j), iv = V.find2(j, j);
+    if (ia < 0 || iv < 0) continue;   // sector cut away: zero block
+    const Block& ba = A.blocks()[ia];
+    const Block& bv = V.blocks()[iv];
+    SU2_CHECK(ba.dim[0] == b.dim[0] && bv.dim[0] == b.dim[1] && ba.dim[1] == bv.dim[1],
+              kErrStructure, "centre: factor dimensions");
+    g.add(psi->data() + b.offset, b.dim[0], b.dim[0], b.dim[1], A.data() + ba.offset, ba.dim[0],
+          0, V.data() + bv.offset, bv.dim[0], 1, ba.dim[1]);
+  }
+  g.run(c);
+  std::vector<HeffTerm> terms = heff_terms(il, ir, *El, *Er);
+  std::unique_ptr<HeffPlan> plan(new HeffPlan(c, *psi, El->H.get(), Er->H.get(), terms));
+  SU2_HIP(hipStreamSynchronize(c.stream()));
+  return {std::move(plan), std::move(psi)};
 }
 
 // The reference's step order as a cursor (dmrg_su2_omp.cpp:20-157), so that a

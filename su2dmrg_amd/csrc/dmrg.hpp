@@ -19,6 +19,7 @@
 
 #include <map>
 #include <memory>
+#include <utility>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -112,6 +113,7 @@ struct DmrgStats {
   double t_matvec32 = 0;
   double heff_flops32 = 0;
   double max_fp64_correction = 0;
+  long long svd_degraded = 0;  // SVDs accepted at the sweep ceiling (see jacobi.hpp)
 };
 
 class Dmrg {
@@ -136,6 +138,9 @@ class Dmrg {
   // warmup(), or after load() and before the next sweep.
   void add_lower_state(const std::string& dir, int file_dcut, int ky, int exci);
   int lower_states() const { return (int)lower_.size(); }
+  // The two-site H_eff (without excited-state penalties) and the wavefunction
+  // at the centre bond of the current state, after warmup() or sweep().
+  std::pair<std::unique_ptr<HeffPlan>, std::unique_ptr<Tensor>> centre();
 
   // Step cursor over the reference's step order (dmrg_su2_omp.cpp:20-157):
   // begin a phase (the warm-up or one sweep), then take its two-site solves

@@ -696,7 +696,8 @@ struct DevMem {
 }  // namespace
 
 int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol, int max_sweeps,
-                       int block_cols, int inner_sweeps) {
+                       int block_cols, int inner_sweeps, double* worst_out) {
+  if (worst_out) *worst_out = 0.0;
   hipStream_t st = c.stream();
   const int nb = (int)blocks.size();
   if (nb == 0) return 0;
@@ -904,10 +905,15 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
       }
     }
   }
-  // out of sweeps: still accept columns orthogonal to 1e-10 of their norms
-  SU2_CHECK(converged || worst <= 1e-10, kErrNumeric,
+  // Out of sweeps: the columns are accepted when orthogonal to 1e-6 of their
+  // norms -- the singular values then carry errors of order 1e-12 relative
+  // (second order in the residual overlaps), harmless for a truncation --
+  // and the shortfall is reported through *worst_out; beyond that it is an
+  // error.
+  SU2_CHECK(converged || worst <= 1e-6, kErrNumeric,
             "jacobi svd: no convergence after %d sweeps (worst |w_i.w_j|/|w_i||w_j| = %.2e)",
             max_sweeps, worst);
+  if (worst_out) *worst_out = converged ? 0.0 : worst;
 
   // singular values in ascending order, sign convention on V (W follows),
   // results into the caller's buffers -- for every block in a few launches

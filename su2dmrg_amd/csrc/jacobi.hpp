@@ -45,14 +45,17 @@ struct JacobiBlock {
                            // each other's bits
 };
 
-// Decomposes every block and returns the number of sweeps it took; throws
-// kErrNumeric when max_sweeps pass without convergence.  W and V columns are sorted ascending by singular value; each
+// Decomposes every block and returns the number of sweeps it took.  W and V columns are sorted ascending by singular value; each
 // column of V has its first entry with |v| >= max|v|/2 made positive (the
 // LAPACK routine the reference calls leaves signs open), W's columns follow.
 // block_cols: 16 (pairs of 32 columns, one wave per pair) or 32 (pairs of 64
 // columns, eight waves per pair); inner_sweeps: 1..3 inner Jacobi sweeps per
 // visit of a pair.
+// worst_out (nullable) receives 0 when every block converged, else the
+// largest remaining |w_i.w_j| / |w_i||w_j| (accepted up to 1e-6, beyond it
+// the call throws).
 int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol = 1e-13,
-                       int max_sweeps = 40, int block_cols = 32, int inner_sweeps = 1);
+                       int max_sweeps = 60, int block_cols = 32, int inner_sweeps = 1,
+                       double* worst_out = nullptr);
 
 }  // namespace su2

@@ -163,7 +163,7 @@ std::vector<SvdResult> truncated_svd_batch(Context& c, const std::vector<SvdJob>
   // the batch gives every wavefunction the bits it has by itself.
   std::vector<std::vector<std::vector<double>>> wv(jobs.size());
   int jacobi_sweeps = 0;
-  double t_jacobi = 0;
+  double t_jacobi = 0, jacobi_worst = 0;
   {
     std::vector<JacobiBlock> jb;
     std::vector<std::pair<size_t, size_t>> idx;
@@ -187,7 +187,8 @@ std::vector<SvdResult> truncated_svd_batch(Context& c, const std::vector<SvdJob>
       }
     }
     const auto tj = std::chrono::steady_clock::now();
-    jacobi_sweeps = jacobi_svd_batched(c, jb, 1e-13, 40, c.svd_block_cols, c.svd_inner_sweeps);
+    jacobi_sweeps = jacobi_svd_batched(c, jb, 1e-13, 60, c.svd_block_cols, c.svd_inner_sweeps,
+                                       &jacobi_worst);
     t_jacobi = std::chrono::duration<double>(std::chrono::steady_clock::now() - tj).count();
     for (size_t t = 0; t < jb.size(); ++t) {
       std::vector<double>& w = wv[idx[t].first][idx[t].second];
@@ -218,6 +219,7 @@ std::vector<SvdResult> truncated_svd_batch(Context& c, const std::vector<SvdJob>
     out.push_back(svd_cut(c, *jobs[q].psi, secs[q], jobs[q].max_dcut));
     out.back().jacobi_sweeps = jacobi_sweeps;
     out.back().t_jacobi = t_jacobi / jobs.size();
+    out.back().jacobi_worst = jacobi_worst;
   }
   SU2_HIP(hipStreamSynchronize(c.stream()));
   return out;

@@ -31,6 +31,8 @@ struct SvdResult {
   std::vector<double> wout;
   int jacobi_sweeps = 0;      // one-sided Jacobi sweeps the sectors took
   double t_jacobi = 0;        // seconds inside the Jacobi SVD
+  double jacobi_worst = 0;    // 0, or the residual overlap accepted after the
+                              // sweep ceiling (no convergence to 1e-13)
 };
 
 SvdResult truncated_svd(Context& c, const Tensor& psi, double p1, double p2, int max_dcut);

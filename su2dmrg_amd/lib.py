@@ -41,6 +41,7 @@ _SIG = {
     "su2_ctx_memory": (_i, [_p, _llp, _llp, _llp, _i]),
     "su2_tensor_create": (_i, [_p, _i, _ip, _ip, _ip, _ip, _i, _ip, C.POINTER(_p)]),
     "su2_tensor_destroy": (_i, [_p, _p]),
+    "su2_tensor_like": (_i, [_p, _p, C.POINTER(_p)]),
     "su2_tensor_size": (_i, [_p, _llp, _ip]),
     "su2_tensor_block": (_i, [_p, _i, _ip, _ip, _ip, _llp]),
     "su2_tensor_upload": (_i, [_p, _p, _dp, _ll]),
@@ -81,6 +82,7 @@ _SIG = {
     "su2_dmrg_stats": (_i, [_p, _dp, _i]),
     "su2_dmrg_set_matvec_precision": (_i, [_p, _p, _i, _d]),
     "su2_dmrg_set_graph": (_i, [_p, _p, _i]),
+    "su2_dmrg_centre": (_i, [_p, _p, C.POINTER(_p), C.POINTER(_p)]),
     "su2_batch_create": (_i, [_p, C.POINTER(_p)]),
     "su2_batch_destroy": (_i, [_p, _p]),
     "su2_batch_add": (_i, [_p, _p, _p, _i, _ip, _ip, _d, _d, _ip]),
@@ -246,6 +248,12 @@ class Tensor:
             self.close()
         except Exception:
             pass
+
+    def like(self, nleg=2):
+        """A zeroed tensor of the same structure (su2_tensor_like)."""
+        h = _p()
+        self.ctx.check(lib.su2_tensor_like(self.ctx.h, self.h, C.byref(h)))
+        return _wrap_tensor(self.ctx, h, nleg=nleg)
 
     # Lanczos algebra
     def copy_from(self, other):
@@ -562,6 +570,17 @@ class Dmrg:
     def set_graph(self, on=True):
         self.ctx.check(lib.su2_dmrg_set_graph(self.ctx.h, self.h, int(bool(on))))
 
+    def centre(self):
+        """(HeffPlan, wavefunction Tensor) of the centre bond of the current state."""
+        hp, ht = _p(), _p()
+        self.ctx.check(lib.su2_dmrg_centre(self.ctx.h, self.h, C.byref(hp), C.byref(ht)))
+        plan = HeffPlan.__new__(HeffPlan)
+        plan.ctx = self.ctx
+        plan.h = hp
+        plan._keep = ()
+        psi = _wrap_tensor(self.ctx, ht, nleg=2)
+        return plan, psi
+
     def set_matvec_precision(self, bits, min_flops=-1.0):
         """64: FP64 matvecs; 32: FP32 MFMA Krylov matvecs with FP64 checks, for
         solves whose matvec has at least `min_flops` FLOPs (< 0: default)."""
@@ -569,11 +588,11 @@ class Dmrg:
                                                          float(min_flops)))
 
     def stats(self):
-        out = (C.c_double * 16)()
-        _check_global(lib.su2_dmrg_stats(self.h, out, 16))
+        out = (C.c_double * 17)()
+        _check_global(lib.su2_dmrg_stats(self.h, out, 17))
         keys = ["t_env", "t_lanczos", "t_svd", "t_project", "t_start", "t_matvec",
                 "heff_flops", "matvecs", "steps", "bond_dim", "t_jacobi", "jacobi_sweeps",
-                "matvecs32", "t_matvec32", "heff_flops32", "max_fp64_correction"]
+                "matvecs32", "t_matvec32", "heff_flops32", "max_fp64_correction", "svd_degraded"]
         return dict(zip(keys, list(out)))
 
     def run_schedule(self, schedule, stop=None, min_flops=-1.0):

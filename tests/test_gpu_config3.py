@@ -4,8 +4,10 @@ The headline workload: the warm-up (do_idmrg, dmrg_su2_omp.cpp:20-95) and
 one sweep (dmrg_su2::sweep, dmrg_su2_omp.cpp:98-157) at D = 2000.  No exact
 energy is known at this size, so the test checks properties the sweep must
 have: the kept multiplets never exceed D, the sweep lowers the warm-up
-energy, and within the sweep's last pass (right edge -> centre) the Ritz
-values do not rise by more than the truncation allows.
+energy, within the sweep's last pass (right edge -> centre) the Ritz values
+do not rise by more than the truncation allows, and the centre bond's H_eff
+on the run's own sector dimensions is symmetric to 1e-12 with the sweep
+energy as the Rayleigh quotient of the state.
 """
 import sys
 
@@ -47,5 +49,28 @@ def test_cyl16x8_d2000_warmup_and_sweep():
         rises.append((k, rise, allow))
         assert rise <= allow, (k, last[k - 1], last[k], lhist[k - 1])
     _say(f"  largest rise in the last pass: {max(r[1] for r in rises):.3e}")
+    # the centre bond's H_eff on the run's own sector dimensions: symmetric,
+    # and its Rayleigh quotient is the sweep energy up to the last cut
+    import numpy as np
+    plan, psi = dm.centre()
+    dims = sorted({(b[1][0], b[2][0], b[2][1]) for b in psi.blocks})
+    _say(f"  centre bond sectors (2j, D_left, D_right): {dims}")
+    out = psi.like()
+    plan.apply(psi, out)
+    rq = out.dot(psi) / psi.dot(psi)
+    _say(f"  Rayleigh quotient {rq:.12f}, sweep energy {e_s:.12f}")
+    assert abs(rq - e_s) <= 10 * hist[-1][1] * abs(e_s) + 1e-10 * abs(e_s)
+    rng = np.random.default_rng(5)
+    u, v, hu, hv = psi.like(), psi.like(), psi.like(), psi.like()
+    u.upload(rng.standard_normal(psi.size))
+    v.upload(rng.standard_normal(psi.size))
+    plan.apply(u, hu)
+    plan.apply(v, hv)
+    a, b = u.dot(hv), hu.dot(v)
+    _say(f"  <u|Hv> = {a:.15e}, <Hu|v> = {b:.15e}")
+    assert abs(a - b) < 1e-12 * abs(a)
+    for t in (out, u, v, hu, hv, psi):
+        t.close()
+    plan.close()
     dm.close()
     ctx.close()
