@@ -243,8 +243,11 @@ int su2_recoupling(int kind, const int* labels, double* out);
  * spin 2S = totspin, keeping max_dcut multiplets (dmrg_su2::dmrg_su2,
  * dmrg_su2.cpp:31-310; flags -lx -ly -jcoup -qcoup -boundary -sec -d of
  * main_omp.cpp:42-132), with the J-Q plaquette terms when qcoup != 0
- * (dmrg_su2.cpp:200-232, 635-655); boundary codes 0, 1, 2.  All environments, MPS projectors and Lanczos
- * vectors stay in device memory. */
+ * (dmrg_su2.cpp:200-232, 635-655).  Boundary codes (dmrg_su2.cpp:97-197): 0 square
+ * lattice, periodic in x and y; 1 J1-J3; 2 J1-J2 with coup[3] on the edge
+ * columns' y-bonds; 3 Shastry-Sutherland (square bonds take coup[2] = 1 - |qcoup|,
+ * orthogonal dimers coup[1] = jcoup + qcoup/2).  All environments, MPS
+ * projectors and Lanczos vectors stay in device memory. */
 int su2_dmrg_create(su2_ctx* ctx, int lx, int ly, double jcoup, double qcoup, int bdry,
                     int totspin, int max_dcut, su2_dmrg** out);
 int su2_dmrg_destroy(su2_ctx* ctx, su2_dmrg* d);

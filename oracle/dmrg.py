@@ -91,16 +91,35 @@ class Lattice:
                         x2, y2 = x1, (y1 + 2) % ly
                     else:
                         flag = False
+                elif bdry == 3:
+                    # Shastry-Sutherland (dmrg_su2.cpp:154-170): square-lattice
+                    # bonds plus one diagonal on every other plaquette, the
+                    # diagonals of neighbouring plaquettes orthogonal.  The
+                    # k = 4 case names a pair but no coupling class
+                    # (dmrg_su2.cpp:180-191), so it adds no bond.
+                    if k == 0:
+                        x2, y2 = x1 + 1, y1
+                    elif k == 1:
+                        x2, y2 = x1, (y1 + 1) % ly
+                    elif k == 2 and x1 % 2 == 0 and y1 % 2 == 0:
+                        x2, y2 = x1 + 1, (y1 + 1) % ly
+                    elif k == 3 and x1 % 2 == 1 and y1 % 2 == 0:
+                        x2, y2 = x1 + 1, (y1 + ly - 1) % ly
+                    else:
+                        flag = False
                 else:
                     raise NotImplementedError("boundary code")
                 if not (flag and 0 <= x2 < lx and 0 <= y2 < ly):
                     continue
                 j = site(x2, y2)
+                # coupling class (dmrg_su2.cpp:180-197): 1 -> coup[1],
+                # 2 -> coup[2], 3 -> coup[3]; on the Shastry-Sutherland
+                # lattice the square bonds are class 2, the dimers class 1
                 if bdry == 2 and k == 1 and (x1 == 0 or x1 == lx - 1):
                     v = 3
-                elif (bdry == 2 or bdry == 1) and 2 <= k < 4:
+                elif (bdry == 3 and k < 2) or (bdry in (1, 2) and 2 <= k < 4):
                     v = 2
-                elif k < 2:
+                elif (bdry == 3 and 2 <= k < 4) or (bdry != 3 and k < 2):
                     v = 1
                 else:
                     continue

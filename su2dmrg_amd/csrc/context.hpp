@@ -16,12 +16,14 @@
 
 namespace su2 {
 
-// Device memory cache of one context's stream.  A freed block is held back
-// until the stream has been synchronised after the free (so no kernel or
-// copy still uses it), then serves a request of at least half its size.  A
-// request that only a held-back block would fit synchronises the stream;
-// otherwise new memory comes from hipMalloc.  Cached blocks return to the
-// driver when the pool is destroyed or hipMalloc runs out of memory.
+// Device memory cache of one context's stream.  All device work on pool
+// memory is queued on that one stream, so a block freed on the host can serve
+// the next request at once (stream order runs every earlier kernel and copy
+// that used it first, as in a stream-ordered allocator): a freed block serves
+// any request of at least half its size, and only a request that no cached
+// block fits goes to hipMalloc.  Cached blocks return to the driver when the
+// pool is destroyed or hipMalloc runs out of memory (after a stream wait).
+// Moving the pool to another stream waits for the old one first.
 // Objects that may outlive their context (tensors, GEMM and H_eff plans)
 // share ownership of the pool: once the context is gone (detach), the pool
 // hands back what it holds and frees later blocks with hipFree directly.
@@ -42,13 +44,12 @@ class DevicePool {
   void reset_peak() { peak_bytes_ = live_bytes_; }
 
  private:
-  void recycle();        // synchronise the stream, held-back blocks become free
-  void release_free();   // free blocks back to the driver
+  void drain();          // wait for the stream: no queued work uses a cached block
+  void release_free();   // cached blocks back to the driver (after drain)
   int device_;
   hipStream_t stream_;
   bool detached_ = false;
-  std::multimap<size_t, void*> free_;           // size -> block, reusable now
-  std::vector<std::pair<size_t, void*>> held_;  // freed since the last sync
+  std::multimap<size_t, void*> free_;           // size -> block, reusable in stream order
   std::unordered_map<void*, size_t> live_;      // handed out
   size_t live_bytes_ = 0, peak_bytes_ = 0;
 };

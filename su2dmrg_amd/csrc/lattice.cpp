@@ -10,8 +10,7 @@ namespace su2 {
 Lattice::Lattice(int lx_, int ly_, double jc, double qc, int bd)
     : lx(lx_), ly(ly_), ns(lx_ * ly_), bdry(bd), jcoup(jc), qcoup(qc) {
   SU2_CHECK(lx >= 1 && ly >= 1 && ns >= 4, kErrArgument, "lattice %dx%d too small", lx, ly);
-  SU2_CHECK(bdry == 0 || bdry == 1 || bdry == 2, kErrArgument, "boundary code %d not supported",
-            bdry);
+  SU2_CHECK(bdry >= 0 && bdry <= 3, kErrArgument, "boundary code %d not supported", bdry);
   SU2_CHECK(std::fabs(qcoup) <= 1e-5 || ly >= 2, kErrArgument,
             "plaquette (Q) terms need ly >= 2 (qcoup=%g)", qcoup);
   hmap.assign((size_t)ns * ns, 0);
@@ -32,19 +31,30 @@ Lattice::Lattice(int lx_, int ly_, double jc, double qc, int bd)
         else if (k == 2) { x2 = x1 + 1; y2 = (y1 + 1) % ly; }
         else if (k == 3) { x2 = x1 - 1; y2 = (y1 + 1) % ly; }
         else flag = false;
-      } else {  // bdry == 1, J1-J3
+      } else if (bdry == 1) {  // J1-J3
         if (k == 0) { x2 = x1 + 1; y2 = y1; }
         else if (k == 1) { x2 = x1; y2 = (y1 + 1) % ly; }
         else if (k == 2) { x2 = x1 + 2; y2 = y1; }
         else if (k == 3) { x2 = x1; y2 = (y1 + 2) % ly; }
         else flag = false;
+      } else {  // bdry == 3, Shastry-Sutherland (dmrg_su2.cpp:154-170)
+        // square bonds, plus one diagonal on every other plaquette with the
+        // diagonals of neighbouring plaquettes orthogonal; the reference's
+        // k = 4 pair gets no coupling class there, so it is left out
+        if (k == 0) { x2 = x1 + 1; y2 = y1; }
+        else if (k == 1) { x2 = x1; y2 = (y1 + 1) % ly; }
+        else if (k == 2 && x1 % 2 == 0 && y1 % 2 == 0) { x2 = x1 + 1; y2 = (y1 + 1) % ly; }
+        else if (k == 3 && x1 % 2 == 1 && y1 % 2 == 0) { x2 = x1 + 1; y2 = (y1 + ly - 1) % ly; }
+        else flag = false;
       }
       if (!flag || x2 < 0 || x2 >= lx || y2 < 0 || y2 >= ly) continue;
       const int j = site(x2, y2);
+      // coupling class (dmrg_su2.cpp:180-197): on the Shastry-Sutherland
+      // lattice the square bonds take coup[2] and the dimers coup[1]
       int v = 0;
       if (bdry == 2 && k == 1 && (x1 == 0 || x1 == lx - 1)) v = 3;
-      else if ((bdry == 2 || bdry == 1) && k >= 2 && k < 4) v = 2;
-      else if (k < 2) v = 1;
+      else if ((bdry == 3 && k < 2) || ((bdry == 2 || bdry == 1) && k >= 2 && k < 4)) v = 2;
+      else if ((bdry == 3 && k >= 2 && k < 4) || (bdry != 3 && k < 2)) v = 1;
       if (!v) continue;
       hmap[(size_t)i * ns + j] = v;
       hmap[(size_t)j * ns + i] = v;

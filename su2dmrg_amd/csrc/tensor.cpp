@@ -49,7 +49,6 @@ size_t pool_round(size_t b) {
 DevicePool::~DevicePool() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
-  for (auto& x : held_) (void)hipFree(x.second);
   for (auto& x : free_) (void)hipFree(x.second);
   for (auto& x : live_) (void)hipFree(x.first);
 }
@@ -69,16 +68,9 @@ void* DevicePool::alloc(size_t bytes) {
     return p;
   };
   if (void* p = take()) return p;
-  bool fits = false;
-  for (auto& x : held_) fits = fits || (x.first >= sz && x.first <= 2 * sz);
-  if (fits) {
-    recycle();
-    if (void* p = take()) return p;
-  }
   void* p = nullptr;
   if (hipMalloc(&p, sz) != hipSuccess) {
     (void)hipGetLastError();
-    recycle();
     release_free();
     SU2_HIP(hipMalloc(&p, sz));
   }
@@ -91,7 +83,6 @@ void* DevicePool::alloc(size_t bytes) {
 size_t DevicePool::cached_bytes() const {
   size_t b = 0;
   for (auto& x : free_) b += x.first;
-  for (auto& x : held_) b += x.first;
   return b;
 }
 
@@ -107,13 +98,15 @@ void DevicePool::free(void* p) {
     (void)hipFree(p);
     return;
   }
-  held_.push_back({it->second, p});
+  // reusable at once: later work on this stream runs after the work queued
+  // so far, the last use of this block included
+  free_.insert({it->second, p});
   live_bytes_ -= it->second;
   live_.erase(it);
 }
 
 void DevicePool::set_stream(hipStream_t s) {
-  recycle();   // blocks held back for the old stream are released against it
+  drain();   // cached blocks may still be in use by work on the old stream
   stream_ = s;
 }
 
@@ -122,21 +115,18 @@ void DevicePool::detach() noexcept {
   // wait's status is dropped (su2_ctx_destroy waits and reports it first)
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
-  for (auto& x : held_) free_.insert({x.first, x.second});
-  held_.clear();
-  release_free();
+  for (auto& x : free_) (void)hipFree(x.second);
+  free_.clear();
   stream_ = nullptr;
   detached_ = true;
 }
 
-void DevicePool::recycle() {
-  if (held_.empty()) return;
-  SU2_HIP(hipStreamSynchronize(stream_));
-  for (auto& x : held_) free_.insert({x.first, x.second});
-  held_.clear();
+void DevicePool::drain() {
+  if (stream_ && !free_.empty()) SU2_HIP(hipStreamSynchronize(stream_));
 }
 
 void DevicePool::release_free() {
+  drain();
   for (auto& x : free_) (void)hipFree(x.second);
   free_.clear();
 }

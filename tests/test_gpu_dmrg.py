@@ -146,6 +146,20 @@ def test_dmrg_cylinder_converges_to_ed(ctx, lx, ly, jcoup, twoS):
         assert min(e[-nstep:]) > e_ed - 1e-10 * abs(e_ed)
 
 
+@pytest.mark.parametrize("lx,ly,jcoup,twoS,nsweep", [(4, 4, 0.8, 0, 4), (4, 2, 1.5, 0, 4),
+                                                    (3, 4, 1.5, 2, 10)])
+def test_shastry_sutherland_converges_to_ed(ctx, lx, ly, jcoup, twoS, nsweep):
+    # boundary code 3: square bonds 1, orthogonal dimers jcoup; periodic
+    # cylinders, so only the converged energy is compared
+    from su2dmrg_amd import lib as L
+    _say(f"dmrg Shastry-Sutherland {lx}x{ly} jcoup={jcoup} 2S={twoS} vs ED")
+    lat = od.Lattice(lx, ly, jcoup=jcoup, bdry=3)
+    e_ed = ed.ground_energy(lat, twoS)
+    g = L.Dmrg(ctx, lx, ly, jcoup, 0.0, 3, twoS, 200)
+    g.run(nsweep=nsweep, stop=None)
+    assert abs(g.energies()[-1] - e_ed) < 1e-10 * abs(e_ed)
+
+
 def test_dmrg_config1_4x4_d200(ctx):
     # J1-J2 at J2/J1 = 0.5 on 4 (open) x 4 (periodic), S = 0, D = 200, 5 sweeps
     # with the reference's |dE| < 1e-6 stop rule: exact up to Lanczos convergence

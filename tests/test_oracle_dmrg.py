@@ -27,6 +27,53 @@ def test_lattice_tables_j1j2_cylinder():
             assert lat.fll[i, j] == int(np.any(lat.hmap[j, i + 1:]))
 
 
+def _xy(lat, i):
+    x = i // lat.ly
+    return x, (i % lat.ly if x % 2 == 0 else lat.ly - 1 - i % lat.ly)
+
+
+@pytest.mark.parametrize("lx,ly", [(4, 4), (6, 4), (5, 6)])
+def test_lattice_tables_shastry_sutherland(lx, ly):
+    # Shastry-Sutherland on an open-x, periodic-y cylinder: every square bond
+    # carries coup[2]; the dimers (coup[1]) sit on the plaquettes whose
+    # lower-left corner (px, py) has px + py even, rising on even columns and
+    # falling on odd ones, so diagonally adjacent dimers are orthogonal.
+    lat = od.Lattice(lx, ly, jcoup=1.5, qcoup=0.0, bdry=3)
+    assert lat.coup[1:3] == [1.5, 1.0]
+    sq, dim = set(), set()
+    for i, j, J in lat.bonds():
+        (x1, y1), (x2, y2) = sorted((_xy(lat, i), _xy(lat, j)))
+        if lat.hmap[i, j] == 2:
+            sq.add(((x1, y1), (x2, y2)))
+        else:
+            assert lat.hmap[i, j] == 1 and J == 1.5
+            dim.add(((x1, y1), (x2, y2)))
+    want_sq = {((x, y), (x + 1, y)) for x in range(lx - 1) for y in range(ly)}
+    want_sq |= {tuple(sorted(((x, y), (x, (y + 1) % ly)))) for x in range(lx) for y in range(ly)}
+    assert sq == want_sq
+    want_dim = set()
+    for px in range(lx - 1):
+        for py in range(ly):
+            if (px + py) % 2:
+                continue
+            a, b = ((px, py), (px + 1, (py + 1) % ly)) if px % 2 == 0 else \
+                ((px, (py + 1) % ly), (px + 1, py))
+            want_dim.add(tuple(sorted((a, b))))
+    assert dim == want_dim
+    ends = [s for d in dim for s in d]
+    assert len(ends) == len(set(ends))          # no site in two dimers
+
+
+@pytest.mark.parametrize("lx,ly,jcoup,twoS,nsweep", [(4, 2, 1.5, 0, 4), (3, 4, 1.5, 0, 4),
+                                                    (4, 4, 0.8, 0, 4), (3, 4, 1.5, 2, 10)])
+def test_shastry_sutherland_dmrg_matches_ed(lx, ly, jcoup, twoS, nsweep):
+    lat = od.Lattice(lx, ly, jcoup=jcoup, bdry=3)
+    e_ed = ed.ground_energy(lat, twoS)
+    d = od.DMRG(lat, totspin=twoS, max_dcut=200)
+    d.run(nsweep=nsweep, stop=None)
+    assert abs(d.energies[-1] - e_ed) < 1e-10 * abs(e_ed)
+
+
 def test_recoupling_tables_internal_consistency():
     # transforming then re-transforming a wavefunction across a site is the identity
     for jl in range(0, 9):
@@ -121,16 +168,24 @@ WELL_CONDITIONED = [(4, 2, 1.0, 0.0, 0, 100, 0), (3, 4, 2.0, 0.0, 0, 20, 2),
 def test_trajectory_is_well_conditioned(monkeypatch, lx, ly, jcoup, qcoup, bdry, D, twoS):
     # The GPU run is compared with the oracle energy by energy on these
     # lattices.  That comparison is meaningful only when the trajectory does
-    # not hinge on roundoff: perturb every Gram matrix, and every wavefunction
-    # before its SVD, by 1e-15 relative noise and require every energy to move
-    # by less than 3e-11.  (The symmetric cylinders fail this: their warm-up
-    # keeps sectors of rounding-noise weight, whose direction is the noise.)
+    # not hinge on roundoff: perturb every Gram matrix, every H_eff matvec and
+    # every wavefunction before its SVD by 1e-15 relative noise and require
+    # every energy to move by less than 3e-11, for each of four noise draws.
+    # (The symmetric cylinders fail this: their warm-up keeps sectors of
+    # rounding-noise weight, whose direction is the noise.  So do the
+    # Shastry-Sutherland cylinders tried, 3x4 to 5x4 at dimer couplings 0.6 to
+    # 2.5: one draw in three moves the 3x4 S = 1 warm-up by 0.013 at its third
+    # step.  Those are compared with exact diagonalisation instead.)
     lat = od.Lattice(lx, ly, jcoup=jcoup, qcoup=qcoup, bdry=bdry)
     ref = od.DMRG(lat, totspin=twoS, max_dcut=D)
     ref.run(nsweep=2, stop=None)
+    for seed in (11, 1, 2, 3):
+        _perturbed_run_matches(monkeypatch, lat, ref, twoS, D, np.random.default_rng(seed))
+
+
+def _perturbed_run_matches(monkeypatch, lat, ref, twoS, D, rng):
     eigh = np.linalg.eigh
     svd = od.truncated_svd
-    rng = np.random.default_rng(11)
 
     def noisy(G):
         E = rng.standard_normal(G.shape)
@@ -141,12 +196,23 @@ def test_trajectory_is_well_conditioned(monkeypatch, lx, ly, jcoup, qcoup, bdry,
         psi = {j: b + 1e-15 * sc * rng.standard_normal(b.shape) for j, b in psi.items()}
         return svd(psi, p1, p2, dcut, info)
 
+    lanczos = od.lanczos
+
+    def noisy_lanczos(apply, v0, **kw):
+        def napply(v):
+            w = apply(v)
+            sc = max(np.abs(b).max() for b in w.values())
+            return {j: b + 1e-15 * sc * rng.standard_normal(b.shape) for j, b in w.items()}
+        return lanczos(napply, v0, **kw)
+
     monkeypatch.setattr(od.np.linalg, "eigh", noisy)
     monkeypatch.setattr(od, "truncated_svd", noisy_svd)
+    monkeypatch.setattr(od, "lanczos", noisy_lanczos)
     pert = od.DMRG(lat, totspin=twoS, max_dcut=D)
     pert.run(nsweep=2, stop=None)
     monkeypatch.setattr(od.np.linalg, "eigh", eigh)
     monkeypatch.setattr(od, "truncated_svd", svd)
+    monkeypatch.setattr(od, "lanczos", lanczos)
     np.testing.assert_allclose(pert.energies, ref.energies, rtol=0, atol=3e-11)
 
 
