@@ -20,12 +20,15 @@ legs of the same line (each can be switched off):
   two), with each run's sweep times and the mixed run's |dE/site| against
   the FP64 run;
 * scan: independent J-Q-J2 DMRG jobs on 8x4 and 12x6 cylinders run one after
-  another and then all together in lockstep (su2_batch: one grouped launch
-  per GEMM stage per Lanczos round, HIP-graph replayed), jobs/hour of each;
+  another, then all together in lockstep (su2_batch: one grouped launch per
+  GEMM stage per Lanczos round, HIP-graph replayed), then dealt over
+  --lanes lockstep batches, each on its own stream and host thread
+  (su2_batch_run_lanes); jobs/hour of each;
 * serve: a Poisson stream of mixed jobs (lattices, couplings, bond
   dimensions, ground and excited levels) through the continuous-batching
-  server, jobs/hour and p50/p99 latency against the same jobs served one at
-  a time;
+  server with 16 slots over --lanes lanes, jobs/hour and p50/p99 latency
+  against the same jobs served one at a time and with the 16 slots in one
+  lane;
 * cpu_sweep: the oracle's CPU sweep of a 16x8 cylinder at a small D beside
   the device's sweep of the same lattice and D.
 """
@@ -109,7 +112,26 @@ def cpu_baseline(work, budget_s=20.0, repeats=3):
                       f"timed passes ({', '.join(f'{t:.2f}' for t in times)} s)"}
 
 
-def run_schedule(L, ctx, lx, ly, schedule, label):
+def centre_matvec(L, ctx, dm, reps=10):
+    """The FP64 matvec of the run's own centre-bond H_eff (su2_dmrg_centre),
+    HIP-event timed: its sectors and TFLOP/s beside the synthetic workload's."""
+    plan, psi = dm.centre()
+    out = psi.like()
+    alg, _ = plan.flops()
+    ms, ms1, ms2 = plan.time(psi, out, reps)
+    dims = sorted(((b[1][0], b[2][0], b[2][1]) for b in psi.blocks), key=lambda x: -x[1] * x[2])
+    rec = {"sectors": len(psi.blocks),
+           "largest_blocks_2j_rows_cols": [list(d) for d in dims[:5]],
+           "psi_doubles": psi.size, "gflop": round(alg / 1e9, 3),
+           "ms_per_matvec": round(ms, 4), "stage_ms": [round(ms1, 4), round(ms2, 4)],
+           "tflops": round(alg / (ms * 1e-3) / 1e12, 3)}
+    plan.close()
+    out.close()
+    psi.close()
+    return rec
+
+
+def run_schedule(L, ctx, lx, ly, schedule, label, centre=False):
     """Warm-up then one sweep per (D, bits) entry; per-sweep wall times (device
     synchronised on both sides), energies and the device-side breakdown."""
     dm = L.Dmrg(ctx, lx, ly, 2.0, 0.0, 2, 0, schedule[0][0])
@@ -154,6 +176,11 @@ def run_schedule(L, ctx, lx, ly, schedule, label):
            "final_energy_per_site": sweeps[-1]["energy_per_site"],
            "max_kept": max(h[0] for h in hist),
            "max_fp64_correction": st["max_fp64_correction"]}
+    if centre:
+        out["centre_bond_matvec"] = centre_matvec(L, ctx, dm)
+        c = out["centre_bond_matvec"]
+        log(f"{label}: centre bond {c['sectors']} sectors, {c['gflop']} GFLOP, "
+            f"{c['ms_per_matvec']} ms, {c['tflops']} TFLOP/s")
     dm.close()
     return out
 
@@ -162,7 +189,7 @@ def sweep_legs(L, ctx, args):
     lx, ly = (int(x) for x in args.sweep_lattice.split("x"))
     D = args.sweep_d
     n = args.sweeps
-    fp64 = run_schedule(L, ctx, lx, ly, [(D, 64)] * n, "fp64 run")
+    fp64 = run_schedule(L, ctx, lx, ly, [(D, 64)] * n, "fp64 run", centre=True)
     ramp = [d for d in (500, 1000, 2000) if d < D] + [D]
     ramp = (ramp + [D] * 3)[:max(1, n - 2)]
     mixed_sched = [(d, 32) for d in ramp] + [(D, 64)] * min(2, n)
@@ -191,7 +218,7 @@ def scan_jobs(njobs, dvals, seed=1):
     return out
 
 
-def scan_leg(L, ctx, njobs, nsweep, dvals):
+def scan_leg(L, ctx, njobs, nsweep, dvals, lanes=1):
     jobs = scan_jobs(njobs, dvals)
     sched = lambda D: [(D, 64)] * nsweep  # noqa: E731
 
@@ -223,17 +250,44 @@ def scan_leg(L, ctx, njobs, nsweep, dvals):
     b.close()
     for g, _ in runs:
         g.close()
-    return {"jobs": njobs, "lattices": "8x4 and 12x6", "D": list(dvals),
-            "sweeps_per_job": nsweep,
-            "sequential_s": round(t_seq, 3), "batched_s": round(t_bat, 3),
-            "jobs_per_hour_sequential": round(3600 * njobs / t_seq, 1),
-            "jobs_per_hour_batched": round(3600 * njobs / t_bat, 1),
-            "speedup": round(t_seq / t_bat, 3),
-            "energies_bitwise_equal": same,
-            "batch_stats": {k: round(v, 3) for k, v in st.items()}}
+    out = {"jobs": njobs, "lattices": "8x4 and 12x6", "D": list(dvals),
+           "sweeps_per_job": nsweep,
+           "sequential_s": round(t_seq, 3), "batched_s": round(t_bat, 3),
+           "jobs_per_hour_sequential": round(3600 * njobs / t_seq, 1),
+           "jobs_per_hour_batched": round(3600 * njobs / t_bat, 1),
+           "speedup": round(t_seq / t_bat, 3),
+           "energies_bitwise_equal": same,
+           "batch_stats": {k: round(v, 3) for k, v in st.items()}}
+    if lanes > 1:
+        # the same jobs over several lanes: a batch per context (stream), run
+        # at once from one host thread each, jobs dealt round robin
+        ctxs = [ctx] + [L.Context(ctx.device) for _ in range(lanes - 1)]
+        bs = [L.Batch(c) for c in ctxs]
+        runs = []
+        t0 = time.perf_counter()
+        for i, j in enumerate(jobs):
+            k = i % lanes
+            g = L.Dmrg(ctxs[k], j["lx"], j["ly"], j["jcoup"], j["qcoup"], 2, j["totspin"], j["D"])
+            runs.append((k, g, bs[k].add(g, sched(j["D"]))))
+        L.run_lanes(bs)
+        for c in ctxs:
+            c.sync()
+        t_lan = time.perf_counter() - t0
+        same_l = all(bs[k].energies(rid)[1][-1] == e for (k, _, rid), e in zip(runs, seq))
+        for _, g, _ in runs:
+            g.close()
+        for b in bs:
+            b.close()
+        for c in ctxs[1:]:
+            c.close()
+        out.update({"lanes": lanes, "lanes_s": round(t_lan, 3),
+                    "jobs_per_hour_lanes": round(3600 * njobs / t_lan, 1),
+                    "speedup_lanes": round(t_seq / t_lan, 3),
+                    "energies_bitwise_equal_lanes": same_l})
+    return out
 
 
-def serve_leg(L, ctx, njobs, rate_per_s, lattices, dvals, seed=3):
+def serve_leg(L, ctx, njobs, rate_per_s, lattices, dvals, seed=3, lanes=1):
     import numpy as np
     rng = np.random.default_rng(seed)
     jobs = []
@@ -247,37 +301,37 @@ def serve_leg(L, ctx, njobs, rate_per_s, lattices, dvals, seed=3):
         jobs.append(L.Job(lx, ly, jcoup=1.0 / s, qcoup=q / s, max_dcut=D, exci=exci, nsweep=1))
     arrivals = np.cumsum(rng.exponential(1.0 / rate_per_s, njobs))
     budget = 0.8 * 288e9
-    with tempfile.TemporaryDirectory() as d:
-        srv = L.Server(ctx, budget, 16, d)
-        ids = [srv.submit(j, float(t)) for j, t in zip(jobs, arrivals)]
-        srv.run()
-        res = [srv.result(i) for i in ids]
-        st = srv.stats()
-        srv.close()
-    with tempfile.TemporaryDirectory() as d:
-        # the same stream, one job at a time (plain sequential GPU runs)
-        srv = L.Server(ctx, budget, 1, d)
-        ids = [srv.submit(j, float(t)) for j, t in zip(jobs, arrivals)]
-        srv.run()
-        res1 = [srv.result(i) for i in ids]
-        st1 = srv.stats()
-        srv.close()
-    lat = np.array([r["latency"] for r in res])
-    lat1 = np.array([r["latency"] for r in res1])
-    same = all(a["energy"] == b["energy"] for a, b in zip(res, res1))
-    return {"jobs": njobs, "poisson_rate_per_s": rate_per_s,
-            "lattices": [f"{a}x{b}" for a, b in lattices], "D": list(dvals),
-            "excited_share": 0.5, "slots": 16, "budget_bytes": budget,
-            "wall_s": round(st["wall"], 3), "max_concurrent": int(st["max_concurrent"]),
-            "jobs_per_hour": round(3600 * njobs / st["wall"], 1),
-            "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
-            "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
-            "sequential": {"wall_s": round(st1["wall"], 3),
-                           "jobs_per_hour": round(3600 * njobs / st1["wall"], 1),
-                           "p50_latency_s": round(float(np.percentile(lat1, 50)), 3),
-                           "p99_latency_s": round(float(np.percentile(lat1, 99)), 3)},
-            "throughput_ratio": round(st1["wall"] / st["wall"], 3),
-            "energies_bitwise_equal_to_sequential": same}
+
+    def serve(slots, nl):
+        with tempfile.TemporaryDirectory() as d:
+            srv = L.Server(ctx, budget, slots, d, lanes=nl)
+            ids = [srv.submit(j, float(t)) for j, t in zip(jobs, arrivals)]
+            srv.run()
+            res = [srv.result(i) for i in ids]
+            st = srv.stats()
+            srv.close()
+        lat = np.array([r["latency"] for r in res])
+        return res, {"wall_s": round(st["wall"], 3), "max_concurrent": int(st["max_concurrent"]),
+                     "jobs_per_hour": round(3600 * njobs / st["wall"], 1),
+                     "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
+                     "p99_latency_s": round(float(np.percentile(lat, 99)), 3)}
+
+    res, main = serve(16, lanes)            # 16 slots over the lanes
+    res1, seq = serve(1, 1)                 # the same stream one job at a time
+    out = {"jobs": njobs, "poisson_rate_per_s": rate_per_s,
+           "lattices": [f"{a}x{b}" for a, b in lattices], "D": list(dvals),
+           "excited_share": 0.5, "slots": 16, "lanes": lanes, "budget_bytes": budget}
+    out.update(main)
+    out["sequential"] = seq
+    if lanes > 1:
+        resb, one = serve(16, 1)            # the 16 slots in one lane
+        out["one_lane"] = one
+        out["one_lane"]["throughput_ratio"] = round(seq["wall_s"] / one["wall_s"], 3)
+        res1 = res1 if all(a["energy"] == b["energy"] for a, b in zip(resb, res1)) else None
+    out["throughput_ratio"] = round(seq["wall_s"] / main["wall_s"], 3)
+    out["energies_bitwise_equal_to_sequential"] = res1 is not None and all(
+        a["energy"] == b["energy"] for a, b in zip(res, res1))
+    return out
 
 
 def cpu_sweep_leg(L, ctx, lx, ly, D):
@@ -323,6 +377,9 @@ def main():
     ap.add_argument("--sweep-d", type=int, default=2000)
     ap.add_argument("--sweeps", type=int, default=3,
                     help="sweeps of each config-3 run (the mixed run: ramp, then two FP64)")
+    ap.add_argument("--lanes", type=int, default=4,
+                    help="lanes (contexts with their own stream and host thread) of the scan "
+                         "and serve legs")
     ap.add_argument("--scan-jobs", type=int, default=32)
     ap.add_argument("--scan-sweeps", type=int, default=1)
     ap.add_argument("--scan-d", default="300,500", help="bond dimensions drawn by the scan jobs")
@@ -411,17 +468,22 @@ def main():
     if "scan" in legs:
         log(f"scan: {args.scan_jobs} jobs, sequential then batched")
         scan = scan_leg(L, ctx, args.scan_jobs, args.scan_sweeps,
-                        [int(x) for x in args.scan_d.split(",")])
+                        [int(x) for x in args.scan_d.split(",")], lanes=args.lanes)
         log(f"scan: {scan['jobs_per_hour_sequential']} -> {scan['jobs_per_hour_batched']} jobs/h "
             f"({scan['speedup']}x), bitwise equal: {scan['energies_bitwise_equal']}")
+        if args.lanes > 1:
+            log(f"scan: {args.lanes} lanes {scan['jobs_per_hour_lanes']} jobs/h "
+                f"({scan['speedup_lanes']}x), bitwise equal: {scan['energies_bitwise_equal_lanes']}")
     serve = None
     if "serve" in legs:
         log(f"serve: {args.serve_jobs} jobs at {args.serve_rate}/s")
         serve = serve_leg(L, ctx, args.serve_jobs, args.serve_rate,
                           [tuple(int(y) for y in x.split("x")) for x in args.serve_lattices.split(",")],
-                          [int(x) for x in args.serve_d.split(",")])
-        log(f"serve: {serve['jobs_per_hour']} jobs/h, p50 {serve['p50_latency_s']} s, p99 "
-            f"{serve['p99_latency_s']} s; sequential {serve['sequential']['jobs_per_hour']} jobs/h")
+                          [int(x) for x in args.serve_d.split(",")], lanes=args.lanes)
+        log(f"serve: {serve['jobs_per_hour']} jobs/h ({args.lanes} lanes), p50 "
+            f"{serve['p50_latency_s']} s, p99 {serve['p99_latency_s']} s; sequential "
+            f"{serve['sequential']['jobs_per_hour']} jobs/h"
+            + (f"; one lane {serve['one_lane']['jobs_per_hour']} jobs/h" if args.lanes > 1 else ""))
 
     cpu = None
     cpu_sweep = None

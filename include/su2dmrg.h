@@ -353,6 +353,15 @@ int su2_batch_add(su2_ctx* ctx, su2_batch* b, su2_dmrg* d, int nsweep, const int
 int su2_batch_tick(su2_ctx* ctx, su2_batch* b, int* stepped);
 /* Ticks until every run has finished. */
 int su2_batch_run(su2_ctx* ctx, su2_batch* b);
+/* Runs n batches to completion at once, one host thread each ("lanes"); batch
+ * i must belong to ctxs[i] and every context must be distinct (one stream per
+ * lane).  One lane's host work (plan builds, environment scatters, its waits
+ * for Lanczos scalars and Jacobi sweeps) then overlaps the others' device
+ * work.  Each run's results are bit for bit those it gets in one lane.  On
+ * failure the other lanes stop at their next tick; the failing lane's message
+ * is in su2_last_error(ctxs[i]) and su2_last_error(NULL).  No counterpart in
+ * the reference (one job per process). */
+int su2_batch_run_lanes(su2_ctx* const* ctxs, su2_batch* const* bs, int n);
 int su2_batch_set_graph(su2_ctx* ctx, su2_batch* b, int on);
 /* Energy after each finished phase of run id (warm-up first); *finished. */
 int su2_batch_energies(const su2_batch* b, int id, int* finished, double* out, int cap, int* n);
@@ -377,6 +386,11 @@ int su2_server_destroy(su2_ctx* ctx, su2_server* s);
 /* arrival_s: seconds after the start of su2_server_run. */
 int su2_server_submit(su2_ctx* ctx, su2_server* s, const su2_job* job, double arrival_s,
                       int* id);
+/* Lanes (1..8, default 1): the slots are split over this many lanes, each on
+ * its own context (stream) of ctx's device with its own batch driver and host
+ * thread, taking arrived jobs in arrival order under the one memory budget.
+ * A job's energy does not depend on the lane count.  Set before su2_server_run. */
+int su2_server_set_lanes(su2_ctx* ctx, su2_server* s, int lanes);
 /* Serves every submitted job to completion (an idle device waits for the
  * next arrival); verbose 1 logs admissions and completions on stderr. */
 int su2_server_run(su2_ctx* ctx, su2_server* s, int verbose);

@@ -1,9 +1,12 @@
 #include "batch.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <set>
+#include <thread>
 
 #include "common.hpp"
 #include "context.hpp"
@@ -315,6 +318,42 @@ int BatchDriver::tick() {
     runs_[ids[q]].d->complete(std::move(solves[q]), jobs[q].res, std::move(sv[q]));
   t_complete += now() - t0;
   return ns;
+}
+
+// ---------------------------------------------------------------------------
+// lanes
+
+std::vector<std::exception_ptr> run_batches_parallel(const std::vector<BatchDriver*>& drivers) {
+  std::set<const Context*> seen;
+  for (BatchDriver* d : drivers) {
+    SU2_CHECK(d != nullptr, kErrArgument, "lanes: null batch");
+    SU2_CHECK(seen.insert(&d->context()).second, kErrArgument,
+              "lanes: two batches on one context (each lane needs its own stream)");
+  }
+  std::vector<std::exception_ptr> err(drivers.size());
+  std::atomic<bool> failed{false};
+  auto lane = [&](size_t i) {
+    try {
+      BatchDriver& d = *drivers[i];
+      d.context().make_current();   // the device is per host thread
+      while (!failed.load() && d.tick() > 0) {
+      }
+    } catch (...) {
+      err[i] = std::current_exception();
+      failed.store(true);
+    }
+  };
+  std::vector<std::thread> th;
+  try {
+    for (size_t i = 1; i < drivers.size(); ++i) th.emplace_back(lane, i);
+  } catch (...) {   // no thread: stop the started lanes before reporting
+    failed.store(true);
+    for (auto& t : th) t.join();
+    throw;
+  }
+  if (!drivers.empty()) lane(0);
+  for (auto& t : th) t.join();
+  return err;
 }
 
 }  // namespace su2

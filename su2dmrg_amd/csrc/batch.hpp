@@ -21,6 +21,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <exception>
 #include <map>
 #include <memory>
 #include <string>
@@ -113,6 +114,7 @@ class BatchDriver {
   int graphs_built() const { return heff_.graphs_built(); }
   long long graph_replays() const { return heff_.replays(); }
   long long ticks() const { return ticks_; }
+  Context& context() const { return c_; }
   double t_lanczos = 0, t_svd = 0, t_prepare = 0, t_complete = 0, t_matvec = 0;
 
  private:
@@ -131,5 +133,15 @@ class BatchDriver {
   std::vector<Run> runs_;
   long long ticks_ = 0;
 };
+
+// Runs several drivers to completion at once, one host thread each (lanes).
+// Each driver must be on its own context, so on its own stream: the host
+// work of one lane (plan builds, environment scatters, its waits for Lanczos
+// scalars and Jacobi sweeps) overlaps the device work of the others, and
+// latency-bound launches of different lanes share the chip.  A run's
+// arithmetic is the same in any lane, so its results are bit for bit those
+// of one lane.  The returned vector holds each lane's exception (null if it
+// finished); after a failure the other lanes stop at their next tick.
+std::vector<std::exception_ptr> run_batches_parallel(const std::vector<BatchDriver*>& drivers);
 
 }  // namespace su2

@@ -724,6 +724,42 @@ int su2_batch_run(su2_ctx* ctx, su2_batch* b) {
   });
 }
 
+int su2_batch_run_lanes(su2_ctx* const* ctxs, su2_batch* const* bs, int n) {
+  std::vector<su2::BatchDriver*> drivers;
+  const int rc = guard(nullptr, [&] {
+    SU2_CHECK(n >= 1, su2::kErrArgument, "lanes: %d batches", n);
+    need(ctxs, "ctxs");
+    need(bs, "batches");
+    for (int i = 0; i < n; ++i) {
+      need(ctxs[i], "ctx");
+      need(bs[i], "batch");
+      SU2_CHECK(&bs[i]->b.context() == &ctxs[i]->c, su2::kErrArgument,
+                "lanes: batch %d is not on context %d", i, i);
+      drivers.push_back(&bs[i]->b);
+    }
+  });
+  if (rc != SU2_OK) return rc;
+  std::vector<std::exception_ptr> err;
+  const int rc2 = guard(nullptr, [&] { err = su2::run_batches_parallel(drivers); });
+  if (rc2 != SU2_OK) return rc2;
+  // each failed lane's message goes to its context; the first failure's
+  // status is returned (and its message is the thread's global one)
+  int first = SU2_OK;
+  for (int i = 0; i < n; ++i) {
+    if (!err[i]) continue;
+    const int r = guard(ctxs[i], [&] { std::rethrow_exception(err[i]); });
+    if (first == SU2_OK) first = r;
+  }
+  if (first != SU2_OK) {
+    for (int i = 0; i < n; ++i)
+      if (err[i]) {
+        g_err = ctxs[i]->c.last_error;
+        break;
+      }
+  }
+  return first;
+}
+
 int su2_batch_set_graph(su2_ctx* ctx, su2_batch* b, int on) {
   return guard(ctx, [&] {
     need(b, "batch");
@@ -805,6 +841,13 @@ int su2_server_submit(su2_ctx* ctx, su2_server* s, const su2_job* job, double ar
     j.stop = job->stop;
     const int i = s->s.submit(j, arrival_s);
     if (id) *id = i;
+  });
+}
+
+int su2_server_set_lanes(su2_ctx* ctx, su2_server* s, int lanes) {
+  return guard(ctx, [&] {
+    need(s, "server");
+    s->s.set_lanes(lanes);
   });
 }
 

@@ -22,8 +22,11 @@ namespace su2 {
 // that used it first, as in a stream-ordered allocator): a freed block serves
 // any request of at least half its size, and only a request that no cached
 // block fits goes to hipMalloc.  Cached blocks return to the driver when the
-// pool is destroyed or hipMalloc runs out of memory (after a stream wait).
-// Moving the pool to another stream waits for the old one first.
+// pool is destroyed, when hipMalloc runs out of memory, and, largest first
+// down to half the limit, when the cache grows past max(4 GiB, the most ever
+// handed out at once) (each after a stream wait), so that several contexts
+// on one device do not starve each other.  Moving the pool to another
+// stream waits for the old one first.
 // Objects that may outlive their context (tensors, GEMM and H_eff plans)
 // share ownership of the pool: once the context is gone (detach), the pool
 // hands back what it holds and frees later blocks with hipFree directly.
@@ -40,18 +43,19 @@ class DevicePool {
   // bytes handed out now, the most handed out at once, bytes held by the cache
   size_t live_bytes() const { return live_bytes_; }
   size_t peak_bytes() const { return peak_bytes_; }
-  size_t cached_bytes() const;
+  size_t cached_bytes() const { return cached_bytes_; }
   void reset_peak() { peak_bytes_ = live_bytes_; }
 
  private:
   void drain();          // wait for the stream: no queued work uses a cached block
   void release_free();   // cached blocks back to the driver (after drain)
+  void trim();           // over the cache limit: largest blocks back to the driver
   int device_;
   hipStream_t stream_;
   bool detached_ = false;
   std::multimap<size_t, void*> free_;           // size -> block, reusable in stream order
   std::unordered_map<void*, size_t> live_;      // handed out
-  size_t live_bytes_ = 0, peak_bytes_ = 0;
+  size_t live_bytes_ = 0, peak_bytes_ = 0, high_water_ = 0, cached_bytes_ = 0;
 };
 
 class Context {

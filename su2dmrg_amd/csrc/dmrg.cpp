@@ -34,12 +34,18 @@ Leg with_dir(Leg l, int d) {
   return l;
 }
 
-std::unique_ptr<Tensor> scalar_op(Context& c, const Leg& l) {
-  return std::unique_ptr<Tensor>(new Tensor(&c, {with_dir(l, 1), with_dir(l, -1)}, 0, nullptr));
+// with zb, the new operator is zeroed by zb.run() instead of a memset
+std::unique_ptr<Tensor> scalar_op(Context& c, const Leg& l, ZeroBatch* zb = nullptr) {
+  std::unique_ptr<Tensor> t(
+      new Tensor(&c, {with_dir(l, 1), with_dir(l, -1)}, 0, nullptr, zb == nullptr));
+  if (zb) zb->add(*t);
+  return t;
 }
-std::unique_ptr<Tensor> vector_op(Context& c, const Leg& l) {
-  return std::unique_ptr<Tensor>(
-      new Tensor(&c, {with_dir(l, 1), op_leg(), with_dir(l, -1)}, 0, nullptr));
+std::unique_ptr<Tensor> vector_op(Context& c, const Leg& l, ZeroBatch* zb = nullptr) {
+  std::unique_ptr<Tensor> t(
+      new Tensor(&c, {with_dir(l, 1), op_leg(), with_dir(l, -1)}, 0, nullptr, zb == nullptr));
+  if (zb) zb->add(*t);
+  return t;
 }
 
 double now() {
@@ -214,9 +220,11 @@ std::unique_ptr<Env> Dmrg::enlarge(const Fused& E, const Env& env, const std::ve
   Context& c = c_;
   std::unique_ptr<Env> out(new Env());
   out->bond = E.fused;
-  out->H = scalar_op(c, E.fused);
-  for (int k : keep) out->S[k] = vector_op(c, E.fused);
-  out->S[newsite] = vector_op(c, E.fused);
+  ZeroBatch zb;
+  out->H = scalar_op(c, E.fused, &zb);
+  for (int k : keep) out->S[k] = vector_op(c, E.fused, &zb);
+  out->S[newsite] = vector_op(c, E.fused, &zb);
+  zb.run(c);
   ScatterBatch sb;
   Tensor& Hh = *out->H;
   if (env.H) env_overlap(sb, E, *env.H, Hh);
@@ -315,12 +323,13 @@ std::unique_ptr<Env> Dmrg::project(const Tensor& P, const Env& hat) {
   std::unique_ptr<Env> out(new Env());
   out->bond = P.leg(1);
   std::vector<std::pair<const Tensor*, Tensor*>> ops;
+  ZeroBatch zb;
   if (hat.H) {
-    out->H = scalar_op(c, out->bond);
+    out->H = scalar_op(c, out->bond, &zb);
     ops.push_back({hat.H.get(), out->H.get()});
   }
   for (auto& kv : hat.S) {
-    out->S[kv.first] = vector_op(c, out->bond);
+    out->S[kv.first] = vector_op(c, out->bond, &zb);
     ops.push_back({kv.second.get(), out->S[kv.first].get()});
   }
   for (auto& kv : hat.P)
@@ -328,9 +337,11 @@ std::unique_ptr<Env> Dmrg::project(const Tensor& P, const Env& hat) {
       auto& dst = out->P[kv.first][r.first];
       dst.reset(new Tensor(&c, {with_dir(out->bond, 1), make_leg({r.first}, {1}, 1),
                                 with_dir(out->bond, -1)},
-                           0, nullptr));
+                           0, nullptr, false));
+      zb.add(*dst);
       ops.push_back({r.second.get(), dst.get()});
     }
+  zb.run(c);
   project_ops(c, P, ops);
   stats_.t_project += now() - t0;
   return out;

@@ -38,9 +38,12 @@ Leg with_dir(Leg l, int d) {
   return l;
 }
 
-std::unique_ptr<Tensor> rank_op(Context& c, const Leg& l, int k) {
-  return std::unique_ptr<Tensor>(
-      new Tensor(&c, {with_dir(l, 1), make_leg({k}, {1}, 1), with_dir(l, -1)}, 0, nullptr));
+// zeroed by zb.run() (one launch for all of a step's new operators)
+std::unique_ptr<Tensor> rank_op(Context& c, const Leg& l, int k, ZeroBatch& zb) {
+  std::unique_ptr<Tensor> t(new Tensor(
+      &c, {with_dir(l, 1), make_leg({k}, {1}, 1), with_dir(l, -1)}, 0, nullptr, false));
+  zb.add(*t);
+  return t;
 }
 
 int op_rank(const Tensor& t) { return t.nleg() == 3 ? t.leg(1).twoj[0] : 0; }
@@ -119,6 +122,7 @@ void Dmrg::grow_plaquettes(const Fused& E, const Env& env, int i, bool left, Env
   const double sig_site = left ? kSigmaLeft : kSigmaRight;
   const double sig_pair = left ? kSigmaRight : kSigmaLeft;
   ScatterBatch sb;
+  ZeroBatch zb;
   for (int m = 0; m < (int)lat_.plq.size(); ++m) {
     const auto& p = lat_.plq[m];
     if (i < p[0] || i > p[3]) continue;
@@ -134,12 +138,12 @@ void Dmrg::grow_plaquettes(const Fused& E, const Env& env, int i, bool left, Env
                   a);
         const Tensor& Sa = *it->second;
         if (ch == 0) {
-          auto T0 = rank_op(c, E.fused, 0);
+          auto T0 = rank_op(c, E.fused, 0, zb);
           pair_into(sb, E, Sa, sig_pair, 1.0, left, *T0);
           out.P[key][0] = std::move(T0);
         } else {
           for (int K : kRanks) {
-            auto TK = rank_op(c, E.fused, K);
+            auto TK = rank_op(c, E.fused, K, zb);
             site_product_into(sb, E, Sa, sig_site,
                               [&](int jb, int jf, int jbp, int jfp) {
                                 return left ? rc_couple_left(jb, jf, jbp, jfp, K, kPhys)
@@ -156,7 +160,7 @@ void Dmrg::grow_plaquettes(const Fused& E, const Env& env, int i, bool left, Env
                 m, i);
       const auto& prev = pit->second;
       if (i == cc) {
-        auto T2 = rank_op(c, E.fused, 2);
+        auto T2 = rank_op(c, E.fused, 2, zb);
         if (ch == 0) {
           // scalar (x) s_p2: operator_initial with a block operator
           site_product_into(sb, E, *prev.at(0), sig_site,
@@ -183,7 +187,7 @@ void Dmrg::grow_plaquettes(const Fused& E, const Env& env, int i, bool left, Env
       } else {
         for (auto& kv : prev) {
           const int K = kv.first;
-          auto TK = rank_op(c, E.fused, K);
+          auto TK = rank_op(c, E.fused, K, zb);
           site_product_into(sb, E, *kv.second, 1.0,
                             [&](int jb, int jf, int jbp, int jfp) {
                               return left ? rc_op_transform_left(jb, jf, jbp, jfp, K, kPhys)
@@ -195,6 +199,7 @@ void Dmrg::grow_plaquettes(const Fused& E, const Env& env, int i, bool left, Env
       }
     }
   }
+  zb.run(c);
   sb.run(c);
 }
 

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <exception>
 #include <thread>
 
 #include "common.hpp"
@@ -56,11 +57,24 @@ int Server::submit(const JobSpec& s, double arrival_s) {
   return (int)specs_.size() - 1;
 }
 
+struct Server::Lane {
+  Context* c = nullptr;
+  std::unique_ptr<Context> own;   // extra lanes' contexts
+  std::unique_ptr<BatchDriver> drv;
+  std::vector<Slot> slots;
+  long long ticks = 0;
+};
+
+void Server::set_lanes(int n) {
+  SU2_CHECK(n >= 1 && n <= 8, kErrArgument, "server: %d lanes", n);
+  lanes_ = n;
+}
+
 // a new run for the slot's current stage: the ground state, or level k kept
 // orthogonal to the levels below it (read back from the job's directory)
-void Server::start_stage(Slot& s) {
+void Server::start_stage(Lane& l, Slot& s) {
   const JobSpec& j = specs_[s.job];
-  s.d.reset(new Dmrg(c_, j.lx, j.ly, j.jcoup, j.qcoup, j.bdry, j.totspin, j.max_dcut));
+  s.d.reset(new Dmrg(*l.c, j.lx, j.ly, j.jcoup, j.qcoup, j.bdry, j.totspin, j.max_dcut));
   for (int k = 0; k < s.stage; ++k) s.d->add_lower_state(s.dir, 0, 0, k);
   std::vector<RunPhase> prog;
   RunPhase w;
@@ -72,81 +86,152 @@ void Server::start_stage(Slot& s) {
     p.max_dcut = j.max_dcut;
     prog.push_back(p);
   }
-  s.run_id = drv_->add(s.d.get(), prog, j.stop);
+  s.run_id = l.drv->add(s.d.get(), prog, j.stop);
 }
 
-void Server::run() {
-  c_.make_current();
-  drv_.reset(new BatchDriver(c_));
+// One lane: admit arrived jobs into its free slots (arrival order over all
+// lanes, one memory budget), tick its driver, retire finished stages.
+void Server::lane_loop(Lane& l) {
+  l.c->make_current();
   const int nj = (int)specs_.size();
-  std::vector<int> order(nj);
-  for (int i = 0; i < nj; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return arrive_[a] < arrive_[b]; });
-  std::vector<Slot> slots(max_slots_);
-  size_t next = 0;
-  double used = 0;
-  int finished = 0;
-  const double t0 = now();
-  while (finished < nj) {
-    const double t = now() - t0;
-    // admit arrived jobs, in arrival order, while a slot and memory are free
-    while (next < order.size() && arrive_[order[next]] <= t) {
-      const int id = order[next];
-      auto it = std::find_if(slots.begin(), slots.end(), [](const Slot& s) { return s.job < 0; });
-      if (it == slots.end() || used + res_[id].footprint > budget_) break;
-      ++next;
-      Slot& s = *it;
-      s.job = id;
-      s.stage = 0;
-      char buf[64];
-      std::snprintf(buf, sizeof buf, "/job%d", id);
-      s.dir = workdir_ + buf;
-      if (specs_[id].exci > 0) mkdir(s.dir.c_str(), 0755);
-      used += res_[id].footprint;
-      res_[id].t_admit = now() - t0;
-      start_stage(s);
-      if (verbose_) std::fprintf(stderr, "server: t=%.2fs admit job %d\n", now() - t0, id);
+  for (;;) {
+    std::vector<Slot*> admitted;
+    double wait = 0;
+    bool more = false;
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      if (failed_ || finished_ == nj) return;
+      const double t = now() - t0_;
+      while (next_ < order_.size() && arrive_[order_[next_]] <= t) {
+        const int id = order_[next_];
+        auto it = std::find_if(l.slots.begin(), l.slots.end(),
+                               [](const Slot& s) { return s.job < 0; });
+        if (it == l.slots.end() || used_ + res_[id].footprint > budget_) break;
+        ++next_;
+        Slot& s = *it;
+        s.job = id;
+        s.stage = 0;
+        char buf[64];
+        std::snprintf(buf, sizeof buf, "/job%d", id);
+        s.dir = workdir_ + buf;
+        used_ += res_[id].footprint;
+        res_[id].t_admit = t;
+        ++live_;
+        max_conc_ = std::max(max_conc_, live_);
+        admitted.push_back(&s);
+      }
+      more = next_ < order_.size();
+      if (more) wait = arrive_[order_[next_]] - t;
+    }
+    for (Slot* s : admitted) {
+      if (specs_[s->job].exci > 0) mkdir(s->dir.c_str(), 0755);
+      start_stage(l, *s);
+      if (verbose_) std::fprintf(stderr, "server: t=%.2fs admit job %d\n", now() - t0_, s->job);
     }
     int live = 0;
-    for (auto& s : slots) live += s.job >= 0;
-    max_conc_ = std::max(max_conc_, live);
+    for (auto& s : l.slots) live += s.job >= 0;
     if (live == 0) {
-      // idle until the next arrival
-      if (next < order.size()) {
-        const double wait = arrive_[order[next]] - (now() - t0);
-        if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
-      }
+      if (!more) return;   // nothing left to admit; the other lanes finish theirs
+      // idle until the next arrival (or until another lane frees memory)
+      const double d = std::min(std::max(wait, 0.0), 2e-3);
+      std::this_thread::sleep_for(std::chrono::duration<double>(d > 0 ? d : 1e-4));
       continue;
     }
-    drv_->tick();
-    ++ticks_;
+    l.drv->tick();
+    ++l.ticks;
     // retire finished stages: next level, or the job is done
-    for (auto& s : slots) {
-      if (s.job < 0 || !drv_->finished(s.run_id)) continue;
+    for (auto& s : l.slots) {
+      if (s.job < 0 || !l.drv->finished(s.run_id)) continue;
       const JobSpec& j = specs_[s.job];
       JobResult& r = res_[s.job];
-      const std::vector<double>& en = drv_->phase_energies(s.run_id);
+      const std::vector<double>& en = l.drv->phase_energies(s.run_id);
       r.energy = en.empty() ? 0 : en.back();
       r.stage = s.stage + 1;
       if (s.stage < j.exci) {
         s.d->save(s.dir, 0, s.stage);
         ++s.stage;
-        start_stage(s);
+        start_stage(l, s);
         continue;
       }
-      r.t_finish = now() - t0;
-      r.done = true;
-      used -= r.footprint;
-      ++finished;
+      s.d.reset();
+      {
+        std::lock_guard<std::mutex> g(mu_);
+        r.t_finish = now() - t0_;
+        r.done = true;
+        used_ -= r.footprint;
+        ++finished_;
+        --live_;
+      }
       if (verbose_)
         std::fprintf(stderr, "server: t=%.2fs job %d done, E=%.12f\n", r.t_finish, s.job,
                      r.energy);
-      s.d.reset();
       s.job = -1;
       s.run_id = -1;
     }
   }
-  wall_ = now() - t0;
+}
+
+void Server::run() {
+  c_.make_current();
+  const int nj = (int)specs_.size();
+  order_.resize(nj);
+  for (int i = 0; i < nj; ++i) order_[i] = i;
+  std::stable_sort(order_.begin(), order_.end(),
+                   [&](int a, int b) { return arrive_[a] < arrive_[b]; });
+  next_ = 0;
+  used_ = 0;
+  finished_ = 0;
+  live_ = 0;
+  failed_ = false;
+  max_conc_ = 0;
+  const int nl = std::min(lanes_, max_slots_);
+  std::vector<std::unique_ptr<Lane>> lanes;
+  for (int k = 0; k < nl; ++k) {
+    std::unique_ptr<Lane> l(new Lane());
+    if (k == 0) {
+      l->c = &c_;
+    } else {
+      l->own.reset(new Context(c_.device()));
+      l->c = l->own.get();
+    }
+    l->drv.reset(new BatchDriver(*l->c));
+    l->slots.resize(max_slots_ / nl + (k < max_slots_ % nl ? 1 : 0));
+    lanes.push_back(std::move(l));
+  }
+  std::vector<std::exception_ptr> err(nl);
+  auto go = [&](int k) {
+    try {
+      lane_loop(*lanes[k]);
+    } catch (...) {
+      err[k] = std::current_exception();
+      std::lock_guard<std::mutex> g(mu_);
+      failed_ = true;
+    }
+  };
+  t0_ = now();
+  std::vector<std::thread> th;
+  try {
+    for (int k = 1; k < nl; ++k) th.emplace_back(go, k);
+  } catch (...) {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      failed_ = true;
+    }
+    for (auto& t : th) t.join();
+    throw;
+  }
+  go(0);
+  for (auto& t : th) t.join();
+  wall_ = now() - t0_;
+  ticks_ = 0;
+  for (auto& l : lanes) ticks_ += l->ticks;
+  // each lane's runs, then its driver, before its context
+  for (auto& l : lanes) {
+    for (auto& s : l->slots) s.d.reset();
+    l->drv.reset();
+  }
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
 }
 
 }  // namespace su2

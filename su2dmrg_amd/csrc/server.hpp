@@ -11,13 +11,18 @@
 //     advance together, one two-site solve each per BatchDriver tick (one
 //     batched Lanczos and one batched SVD for all of them);
 //   * a finished job frees its slot and memory at once; a job arriving while
-//     others run joins at the next tick.
+//     others run joins at the next tick;
+//   * with several lanes, the slots are split over lanes, each with its own
+//     context (stream), BatchDriver and host thread; a lane takes the next
+//     arrived job that fits when one of its slots is free, and the lanes'
+//     host work overlaps each other's device work (run_batches_parallel).
 // An excited-state job (level k > 0) runs its ground state and the levels
 // below k first, each stage saving its state in the reference's file format
 // for the next (add_lower_state), all inside its slot.
 #pragma once
 
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -57,13 +62,16 @@ class Server {
   // Runs the loop until every submitted job has finished; arrival times
   // count from the start of this call (an idle device waits for the next).
   void run();
+  // Lanes (1..8, default 1); the extra lanes' contexts are made on the
+  // device of the server's context for each run().
+  void set_lanes(int n);
+  int lanes() const { return lanes_; }
   const JobResult& result(int id) const { return res_.at(id); }
   int njobs() const { return (int)specs_.size(); }
   double wall() const { return wall_; }
   long long ticks() const { return ticks_; }
   int max_concurrent() const { return max_conc_; }
   void set_verbose(int v) { verbose_ = v; }
-  BatchDriver* driver() { return drv_.get(); }
 
  private:
   struct Slot {
@@ -73,19 +81,28 @@ class Server {
     std::unique_ptr<Dmrg> d;
     std::string dir;
   };
-  void start_stage(Slot& s);
+  struct Lane;
+  void start_stage(Lane& l, Slot& s);
+  void lane_loop(Lane& l);
   Context& c_;
   double budget_;
   int max_slots_;
+  int lanes_ = 1;
   std::string workdir_;
   std::vector<JobSpec> specs_;
   std::vector<double> arrive_;
   std::vector<JobResult> res_;
-  std::unique_ptr<BatchDriver> drv_;
   double wall_ = 0;
   long long ticks_ = 0;
   int max_conc_ = 0;
   int verbose_ = 0;
+  // shared by the lanes of one run(), under mu_
+  std::mutex mu_;
+  std::vector<int> order_;
+  size_t next_ = 0;
+  double used_ = 0, t0_ = 0;
+  int finished_ = 0, live_ = 0;
+  bool failed_ = false;
 };
 
 }  // namespace su2

@@ -38,6 +38,7 @@ void Context::set_stream(hipStream_t s) {
 }
 
 namespace {
+constexpr size_t kCacheFloor = size_t(4) << 30;   // cache limit floor: 4 GiB
 // allocation granule: 256 B up to 1 MiB, 64 KiB up to 64 MiB, 2 MiB above
 size_t pool_round(size_t b) {
   const size_t g = b <= (size_t(1) << 20) ? 256 : b <= (size_t(64) << 20) ? (size_t(64) << 10)
@@ -63,7 +64,9 @@ void* DevicePool::alloc(size_t bytes) {
     void* p = it->second;
     live_[p] = it->first;
     live_bytes_ += it->first;
+    cached_bytes_ -= it->first;
     peak_bytes_ = std::max(peak_bytes_, live_bytes_);
+    high_water_ = std::max(high_water_, live_bytes_);
     free_.erase(it);
     return p;
   };
@@ -77,13 +80,8 @@ void* DevicePool::alloc(size_t bytes) {
   live_[p] = sz;
   live_bytes_ += sz;
   peak_bytes_ = std::max(peak_bytes_, live_bytes_);
+  high_water_ = std::max(high_water_, live_bytes_);
   return p;
-}
-
-size_t DevicePool::cached_bytes() const {
-  size_t b = 0;
-  for (auto& x : free_) b += x.first;
-  return b;
 }
 
 void DevicePool::free(void* p) {
@@ -101,8 +99,23 @@ void DevicePool::free(void* p) {
   // reusable at once: later work on this stream runs after the work queued
   // so far, the last use of this block included
   free_.insert({it->second, p});
+  cached_bytes_ += it->second;
   live_bytes_ -= it->second;
   live_.erase(it);
+  if (cached_bytes_ > std::max(kCacheFloor, high_water_)) trim();
+}
+
+void DevicePool::trim() {
+  // runs from free(), which destructors call: no throw; a failed wait
+  // leaves the cache as it is
+  const size_t target = std::max(kCacheFloor, high_water_) / 2;
+  if (stream_ && hipStreamSynchronize(stream_) != hipSuccess) return;
+  while (cached_bytes_ > target && !free_.empty()) {
+    auto it = std::prev(free_.end());
+    cached_bytes_ -= it->first;
+    (void)hipFree(it->second);
+    free_.erase(it);
+  }
 }
 
 void DevicePool::set_stream(hipStream_t s) {
@@ -117,6 +130,7 @@ void DevicePool::detach() noexcept {
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto& x : free_) (void)hipFree(x.second);
   free_.clear();
+  cached_bytes_ = 0;
   stream_ = nullptr;
   detached_ = true;
 }
@@ -129,6 +143,7 @@ void DevicePool::release_free() {
   drain();
   for (auto& x : free_) (void)hipFree(x.second);
   free_.clear();
+  cached_bytes_ = 0;
 }
 
 double* Context::reduce_scratch(size_t n) {
@@ -157,7 +172,8 @@ static long long key_of(const int* twoj, int n) {
   return k;
 }
 
-Tensor::Tensor(Context* ctx, std::vector<Leg> legs, int locspin, const std::vector<char>* present)
+Tensor::Tensor(Context* ctx, std::vector<Leg> legs, int locspin, const std::vector<char>* present,
+               bool zero)
     : ctx_(ctx), legs_(std::move(legs)), locspin_(locspin) {
   const int n = (int)legs_.size();
   SU2_CHECK(n >= 1 && n <= kMaxLegs, kErrArgument, "tensor: %d legs", n);
@@ -201,8 +217,36 @@ Tensor::Tensor(Context* ctx, std::vector<Leg> legs, int locspin, const std::vect
   pool_ = ctx_->pool();
   if (size_ > 0) {
     d_ = ctx_->dev_alloc_n<double>(size_);
-    SU2_HIP(hipMemsetAsync(d_, 0, size_ * sizeof(double), ctx_->stream()));
+    if (zero) SU2_HIP(hipMemsetAsync(d_, 0, size_ * sizeof(double), ctx_->stream()));
   }
+}
+
+void ZeroBatch::add(Tensor& t) {
+  if (t.size() > 0) z_.push_back({t.data(), t.size()});
+}
+
+void ZeroBatch::run(Context& c) {
+  if (z_.empty()) return;
+  std::vector<VecMove> h(z_.size());
+  long long mx = 0;
+  for (size_t i = 0; i < z_.size(); ++i) {
+    h[i] = VecMove{nullptr, z_[i].first, z_[i].second, kZeroF64, 0};
+    mx = std::max(mx, z_[i].second);
+  }
+  // the descriptor block returns to the cache at once; stream order keeps it
+  // until the launch has read it
+  VecMove* d = (VecMove*)c.dev_alloc(h.size() * sizeof(VecMove));
+  try {
+    SU2_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(VecMove), hipMemcpyHostToDevice,
+                           c.stream()));
+    for (size_t i = 0; i < h.size(); i += 65535)
+      dev_move_batch(c, d + i, (int)std::min<size_t>(65535, h.size() - i), mx);
+  } catch (...) {
+    c.dev_free(d);
+    throw;
+  }
+  c.dev_free(d);
+  z_.clear();
 }
 
 Tensor::~Tensor() {

@@ -36,7 +36,10 @@ class DevicePool;
 
 class Tensor {
  public:
-  Tensor(Context* ctx, std::vector<Leg> legs, int locspin, const std::vector<char>* present);
+  // zero = false leaves the blocks unset: the caller zeroes them (ZeroBatch)
+  // before any kernel reads or accumulates into them
+  Tensor(Context* ctx, std::vector<Leg> legs, int locspin, const std::vector<char>* present,
+         bool zero = true);
   ~Tensor();
   Tensor(const Tensor&) = delete;
   Tensor& operator=(const Tensor&) = delete;
@@ -67,6 +70,17 @@ class Tensor {
   double* d_ = nullptr;
   std::shared_ptr<DevicePool> pool_;   // outlives the context if need be
   long long size_ = 0;
+};
+
+// Zero fills of many tensors as one upload and one launch (in place of a
+// memset per tensor), queued on the context's stream by run().
+class ZeroBatch {
+ public:
+  void add(Tensor& t);
+  void run(Context& c);
+
+ private:
+  std::vector<std::pair<double*, long long>> z_;
 };
 
 }  // namespace su2

@@ -88,12 +88,14 @@ _SIG = {
     "su2_batch_add": (_i, [_p, _p, _p, _i, _ip, _ip, _d, _d, _ip]),
     "su2_batch_tick": (_i, [_p, _p, _ip]),
     "su2_batch_run": (_i, [_p, _p]),
+    "su2_batch_run_lanes": (_i, [C.POINTER(_p), C.POINTER(_p), _i]),
     "su2_batch_set_graph": (_i, [_p, _p, _i]),
     "su2_batch_energies": (_i, [_p, _i, _ip, _dp, _i, _ip]),
     "su2_batch_stats": (_i, [_p, _dp, _i]),
     "su2_server_create": (_i, [_p, _d, _i, C.c_char_p, C.POINTER(_p)]),
     "su2_server_destroy": (_i, [_p, _p]),
     "su2_server_submit": (_i, [_p, _p, _p, _d, _ip]),
+    "su2_server_set_lanes": (_i, [_p, _p, _i]),
     "su2_server_run": (_i, [_p, _p, _i]),
     "su2_server_result": (_i, [_p, _i, _dp, _dp, _dp]),
     "su2_server_stats": (_i, [_p, _dp, _i]),
@@ -158,6 +160,7 @@ class Context:
         if rc:
             raise Su2Error(rc, lib.su2_last_error(None).decode())
         self.h = h
+        self.device = device
 
     def check(self, rc):
         if rc:
@@ -697,16 +700,28 @@ class Batch:
             pass
 
 
-class Server:
-    """su2_server: request loop with continuous batching on one device."""
+def run_lanes(batches):
+    """su2_batch_run_lanes: runs the batches at once, one host thread each;
+    every batch must be on its own Context (one stream per lane)."""
+    n = len(batches)
+    ctxs = (_p * n)(*[b.ctx.h for b in batches])
+    hs = (_p * n)(*[b.h for b in batches])
+    _check_global(lib.su2_batch_run_lanes(ctxs, hs, n))
 
-    def __init__(self, ctx, budget_bytes, max_slots, workdir):
+
+class Server:
+    """su2_server: request loop with continuous batching on one device; with
+    lanes > 1 the slots are split over that many streams and host threads."""
+
+    def __init__(self, ctx, budget_bytes, max_slots, workdir, lanes=1):
         self.ctx = ctx
         h = _p()
         ctx.check(lib.su2_server_create(ctx.h, float(budget_bytes), int(max_slots),
                                         os.fsencode(workdir), C.byref(h)))
         self.h = h
         self.n = 0
+        if lanes != 1:
+            ctx.check(lib.su2_server_set_lanes(ctx.h, self.h, int(lanes)))
 
     def submit(self, job, arrival_s=0.0):
         jid = _i()

@@ -23,3 +23,11 @@ def test_null_arguments_are_rejected_without_a_device():
     assert L.lib.su2_tensor_size(None, None, None) == 1
     assert L.lib.su2_heff_flops(None, None, None) == 1
     assert b"NULL" in L.lib.su2_last_error(None)
+
+
+def test_lanes_arguments_are_checked_without_a_device():
+    assert L.lib.su2_batch_run_lanes(None, None, 0) == 1
+    assert b"0 batches" in L.lib.su2_last_error(None)
+    assert L.lib.su2_batch_run_lanes(None, None, 2) == 1
+    assert b"NULL" in L.lib.su2_last_error(None)
+    assert L.lib.su2_server_set_lanes(None, None, 2) == 1

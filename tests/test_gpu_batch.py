@@ -111,3 +111,41 @@ def test_single_run_graph_and_direct_agree(ctx):
     ea, _ = _alone(L, ctx, RUNS[1], graph=True)
     eb, _ = _alone(L, ctx, RUNS[1], graph=False)
     assert np.array_equal(ea, eb)
+
+
+def test_lanes_equal_runs_alone(ctx):
+    # three lanes, each a batch on its own context (stream) and host thread;
+    # the runs are spread over them round robin and run at once
+    from su2dmrg_amd import lib as L
+    _say("runs in 3 lanes vs each alone")
+    alone = [_alone(L, ctx, s) for s in RUNS]
+    ctxs = [ctx, L.Context(0), L.Context(0)]
+    lanes = [L.Batch(c) for c in ctxs]
+    runs = []
+    for i, (lx, ly, j, q, S, sched) in enumerate(RUNS):
+        k = i % 3
+        g = L.Dmrg(ctxs[k], lx, ly, j, q, 2, S, sched[0][0])
+        runs.append((k, g, lanes[k].add(g, sched, min_flops=0.0)))
+    L.run_lanes(lanes)
+    for (ea, ha), (k, g, rid) in zip(alone, runs):
+        fin, ph = lanes[k].energies(rid)
+        assert fin
+        eb = np.array(g.energies())
+        assert np.array_equal(ea, eb), np.max(np.abs(ea - eb))
+        assert ha == g.history()
+        assert ph[-1] == eb[-1]
+    for _, g, _ in runs:
+        g.close()
+    for b in lanes:
+        b.close()
+    for c in ctxs[1:]:
+        c.close()
+
+
+def test_lanes_reject_a_shared_context(ctx):
+    from su2dmrg_amd import lib as L
+    a, b = L.Batch(ctx), L.Batch(ctx)
+    with pytest.raises(L.Su2Error, match="one context"):
+        L.run_lanes([a, b])
+    a.close()
+    b.close()

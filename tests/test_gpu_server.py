@@ -81,6 +81,29 @@ def test_served_jobs_equal_jobs_alone(ctx):
     assert e0 < mixed[2]["energy"] < mixed[4]["energy"]
 
 
+def test_lanes_serve_the_same_energies(ctx):
+    # the same stream of jobs over three lanes (contexts, host threads)
+    from su2dmrg_amd import lib as L
+    rng = np.random.default_rng(7)
+    jobs = _jobs(L)
+    arrivals = np.cumsum(rng.exponential(0.05, len(jobs)))
+    out = {}
+    for lanes in (1, 3):
+        _say(f"serving {len(jobs)} jobs in {lanes} lane(s)")
+        with tempfile.TemporaryDirectory() as d:
+            srv = L.Server(ctx, 8e9, 6, d, lanes=lanes)
+            ids = [srv.submit(j, float(t)) for j, t in zip(jobs, arrivals)]
+            srv.run()
+            out[lanes] = [srv.result(i) for i in ids]
+            st = srv.stats()
+            srv.close()
+        _say(f"  wall {st['wall']:.2f} s, {st['ticks']:.0f} ticks, "
+             f"up to {st['max_concurrent']:.0f} jobs at once")
+    for a, b in zip(out[1], out[3]):
+        assert a["energy"] == b["energy"]
+        assert b["t_arrive"] <= b["t_admit"] + 1e-9 and b["t_admit"] <= b["t_finish"]
+
+
 def test_admission_respects_the_budget(ctx):
     from su2dmrg_amd import lib as L
     j = L.Job(4, 4, jcoup=2.0, max_dcut=30, nsweep=1)
@@ -95,6 +118,16 @@ def test_admission_respects_the_budget(ctx):
         srv.close()
     assert st["max_concurrent"] == 2
     assert len(set(r["energy"] for r in res)) == 1
+    # the same budget over two lanes: still two jobs at a time
+    with tempfile.TemporaryDirectory() as d:
+        srv = L.Server(ctx, 2.5 * fp, 4, d, lanes=2)
+        ids = [srv.submit(j, 0.0) for _ in range(4)]
+        srv.run()
+        st = srv.stats()
+        res2 = [srv.result(i) for i in ids]
+        srv.close()
+    assert st["max_concurrent"] == 2
+    assert all(r["energy"] == res[0]["energy"] for r in res2)
     with pytest.raises(L.Su2Error):
         srv2 = L.Server(ctx, 0.5 * fp, 1, "/tmp")
         srv2.submit(j, 0.0)
