@@ -377,7 +377,7 @@ def main():
     ap.add_argument("--sweep-d", type=int, default=2000)
     ap.add_argument("--sweeps", type=int, default=3,
                     help="sweeps of each config-3 run (the mixed run: ramp, then two FP64)")
-    ap.add_argument("--lanes", type=int, default=4,
+    ap.add_argument("--lanes", type=int, default=6,
                     help="lanes (contexts with their own stream and host thread) of the scan "
                          "and serve legs")
     ap.add_argument("--scan-jobs", type=int, default=32)

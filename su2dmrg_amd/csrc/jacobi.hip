@@ -847,8 +847,8 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
   int sweeps_done = 0;
   for (int sweep = 0; sweep < max_sweeps && !converged; ++sweep) {
     ++sweeps_done;
-    // descending column norms
-    if (sweep > 0) norms();
+    // descending column norms (hn: read back with the previous sweep's
+    // convergence counters, or before the first sweep)
     for (int b = 0; b < nb; ++b) {
       int* p = hp.data() + noff[b];
       const double* w = hn.data() + noff[b];
@@ -893,6 +893,11 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
     SU2_HIP(hipGetLastError());
     SU2_HIP(hipMemcpyAsync(hoff.data(), doff.p, sizeof(unsigned long long) * nb,
                            hipMemcpyDeviceToHost, st));
+    // the column norms for the next sweep's order (or the final one) come
+    // back with the counters: one wait per sweep
+    hipLaunchKernelGGL(colnorm_batched_kernel, dim3((unsigned)ncol), dim3(256), 0, st,
+                       (const BlockDesc*)ddesc.p, (const long long*)dnoff.p, nb, (double*)dn.p);
+    SU2_HIP(hipMemcpyAsync(hn.data(), dn.p, sizeof(double) * ncol, hipMemcpyDeviceToHost, st));
     SU2_HIP(hipStreamSynchronize(st));
     converged = true;
     worst = 0;
@@ -916,8 +921,8 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
   if (worst_out) *worst_out = converged ? 0.0 : worst;
 
   // singular values in ascending order, sign convention on V (W follows),
-  // results into the caller's buffers -- for every block in a few launches
-  norms();
+  // results into the caller's buffers -- for every block in a few launches;
+  // hn holds the final column norms
   std::vector<int> ident(ncol);
   for (int b = 0; b < nb; ++b) {
     JacobiBlock& J = blocks[b];
@@ -961,10 +966,10 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
   }
   {
     // the V moves read the identity permutation, the W moves the sort order
+    // (the second upload of the job table is queued behind the first launch)
     std::swap(dperm.p, dident.p);
     permute(vj, (const double*)dsign.p);
     std::swap(dperm.p, dident.p);
-    SU2_HIP(hipStreamSynchronize(st));   // the job table is reused below
     permute(wj, (const double*)dsign.p);
   }
   if (!moves.empty()) {
@@ -974,9 +979,9 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
     SU2_HIP(hipMemcpyAsync(dm.p, moves.data(), sizeof(VecMove) * moves.size(),
                            hipMemcpyHostToDevice, st));
     dev_move_batch(c, (const VecMove*)dm.p, (int)moves.size(), mx);
-    SU2_HIP(hipStreamSynchronize(st));
   }
-  SU2_HIP(hipStreamSynchronize(st));
+  // no wait: the outputs are used in stream order, and the host results (s)
+  // are already on the host
   SU2_HIP(hipGetLastError());
   return sweeps_done;
 }
