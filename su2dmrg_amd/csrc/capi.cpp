@@ -11,6 +11,7 @@
 #include "cg.hpp"
 #include "common.hpp"
 #include "context.hpp"
+#include "precond.hpp"
 #include "dmrg.hpp"
 #include "heff.hpp"
 #include "recoupling.hpp"
@@ -134,6 +135,15 @@ int su2_ctx_set_svd_tuning(su2_ctx* ctx, int block_cols, int inner_sweeps) {
               "svd inner sweeps %d (1..3)", inner_sweeps);
     ctx->c.svd_block_cols = block_cols;
     ctx->c.svd_inner_sweeps = inner_sweeps;
+  });
+}
+
+int su2_ctx_set_svd_precondition(su2_ctx* ctx, int min_k) {
+  return guard(ctx, [&] {
+    need(ctx, "ctx");
+    SU2_CHECK(min_k == 0 || (min_k >= 2 && min_k <= su2::kPrecondMaxK), su2::kErrArgument,
+              "svd preconditioning from block size %d", min_k);
+    ctx->c.svd_precondition_min_k = min_k;
   });
 }
 

@@ -11,6 +11,7 @@
 #include "blas1.hpp"
 #include "common.hpp"
 #include "context.hpp"
+#include "precond.hpp"
 
 namespace su2 {
 
@@ -604,24 +605,35 @@ __device__ __forceinline__ int col_block(const long long* __restrict__ noff, int
   return lo;
 }
 
-// squared column norms of every block's current W (one workgroup per column)
+// squared column norms of every block's current W into out[0, ncol) and of
+// its V into out[ncol, 2 ncol) (one workgroup per column)
 __global__ __launch_bounds__(256) void colnorm_batched_kernel(const BlockDesc* __restrict__ desc,
                                                               const long long* __restrict__ noff,
-                                                              int nb, double* __restrict__ out) {
-  __shared__ double s[256];
+                                                              int nb, long long ncol,
+                                                              double* __restrict__ out) {
+  __shared__ double s[2][256];
   const long long gc = blockIdx.x;
   const int b = col_block(noff, nb, gc);
   const BlockDesc B = desc[b];
   const double* col = B.W + (gc - noff[b]) * B.L;
-  double a = 0.0;
+  const double* vcol = B.V + (gc - noff[b]) * B.k;
+  double a = 0.0, v = 0.0;
   for (int r = threadIdx.x; r < B.L; r += 256) a += col[r] * col[r];
-  s[threadIdx.x] = a;
+  for (int r = threadIdx.x; r < B.k; r += 256) v += vcol[r] * vcol[r];
+  s[0][threadIdx.x] = a;
+  s[1][threadIdx.x] = v;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    if ((int)threadIdx.x < w) {
+      s[0][threadIdx.x] += s[0][threadIdx.x + w];
+      s[1][threadIdx.x] += s[1][threadIdx.x + w];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[gc] = s[0];
+  if (threadIdx.x == 0) {
+    out[gc] = s[0][0];
+    out[ncol + gc] = s[1][0];
+  }
 }
 
 // dst[:, c] = sign[c] * src[:, perm[c]] for every job (perm and sign hold
@@ -650,7 +662,11 @@ __global__ void permute_batched_kernel(const PermJob* __restrict__ jobs, int nj,
   J.dst[i] = sign[J.col_off + c] * J.src[r + (long long)perm[J.col_off + c] * J.rows];
 }
 
-// col_sign_kernel for the V of every block (desc[b].V, k x k)
+// col_sign_kernel for the V of every block (desc[b].V, k x k), divided by
+// the column's norm: the accumulated rotations leave V's columns off unit
+// length by the rounding of every rotation applied (a few 1e-13 after tens of
+// sweeps over hundreds of columns), and W = A V carries the same factor, so
+// one scaling of both restores the triplet
 __global__ __launch_bounds__(256) void col_sign_batched_kernel(const BlockDesc* __restrict__ desc,
                                                                const long long* __restrict__ noff,
                                                                int nb, double* __restrict__ sign) {
@@ -670,15 +686,27 @@ __global__ __launch_bounds__(256) void col_sign_batched_kernel(const BlockDesc* 
   }
   const double half = 0.5 * sv[0];
   int first = 0x7fffffff;
-  for (int r = threadIdx.x; r < rows; r += 256)
+  double n2 = 0.0;
+  for (int r = threadIdx.x; r < rows; r += 256) {
     if (fabs(col[r]) >= half && r < first) first = r;
+    n2 += col[r] * col[r];
+  }
   si[threadIdx.x] = first;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) si[threadIdx.x] = min(si[threadIdx.x], si[threadIdx.x + w]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) sign[gc] = (si[0] < rows && col[si[0]] < 0.0) ? -1.0 : 1.0;
+  sv[threadIdx.x] = n2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sv[threadIdx.x] += sv[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double inv = sv[0] > 0.0 ? 1.0 / sqrt(sv[0]) : 1.0;
+    sign[gc] = (si[0] < rows && col[si[0]] < 0.0) ? -inv : inv;
+  }
 }
 
 unsigned nblk(long long n, int t) { return (unsigned)std::max<long long>(1, (n + t - 1) / t); }
@@ -696,7 +724,8 @@ struct DevMem {
 }  // namespace
 
 int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol, int max_sweeps,
-                       int block_cols, int inner_sweeps, double* worst_out) {
+                       int block_cols, int inner_sweeps, double* worst_out,
+                       int precondition_min_k) {
   if (worst_out) *worst_out = 0.0;
   hipStream_t st = c.stream();
   const int nb = (int)blocks.size();
@@ -737,10 +766,10 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
                        K[b]);
   }
   SU2_HIP(hipGetLastError());
-  DevMem dn(c, sizeof(double) * ncol), dperm(c, sizeof(int) * ncol),
+  DevMem dn(c, sizeof(double) * 2 * ncol), dperm(c, sizeof(int) * ncol),
       done(c, sizeof(double) * ncol), dsign(c, sizeof(double) * ncol),
       dnoff(c, sizeof(long long) * (nb + 1)), dpj(c, sizeof(PermJob) * 2 * nb);
-  std::vector<double> hn(ncol);
+  std::vector<double> hn(2 * ncol);   // W then V squared column norms
   std::vector<int> hp(ncol);
   {
     std::vector<double> ones(ncol, 1.0);
@@ -761,8 +790,10 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
   // squared column norms of the current W of every block, to the host
   auto norms = [&]() {
     hipLaunchKernelGGL(colnorm_batched_kernel, dim3((unsigned)ncol), dim3(256), 0, st,
-                       (const BlockDesc*)ddesc.p, (const long long*)dnoff.p, nb, (double*)dn.p);
-    SU2_HIP(hipMemcpyAsync(hn.data(), dn.p, sizeof(double) * ncol, hipMemcpyDeviceToHost, st));
+                       (const BlockDesc*)ddesc.p, (const long long*)dnoff.p, nb, (long long)ncol,
+                       (double*)dn.p);
+    SU2_HIP(hipMemcpyAsync(hn.data(), dn.p, sizeof(double) * 2 * ncol, hipMemcpyDeviceToHost,
+                           st));
     SU2_HIP(hipStreamSynchronize(st));
   };
   // src -> dst column moves of W and V (or the given jobs) in one launch
@@ -787,60 +818,129 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
     for (int i = 0; i < K[b]; ++i) g += hn[noff[b] + i];
     desc[b].normF = std::sqrt(g);
   }
-  // round-robin schedule of column-block pairs, merged across blocks by round
-  std::vector<std::vector<PairJob>> rounds;
-  for (int b = 0; b < nb; ++b) {
-    const int ncb = (K[b] + CB - 1) / CB;
-    auto width = [&](int i) { return i >= ncb ? 0 : std::min(CB, K[b] - i * CB); };
-    if (ncb == 1) {
-      if (rounds.empty()) rounds.resize(1);
-      if (K[b] >= 2) rounds[0].push_back(PairJob{b, 0, K[b], 0, 0, 1});
-      continue;
+  // descending column norms of every block (de Rijk's ordering), W and V
+  // permuted alike
+  // blocks still iterating: a block leaves after its first sweep without a
+  // rotation (later sweeps would leave it unchanged: its columns stay in
+  // order and no pair rotates), so converged blocks cost no further rounds
+  std::vector<char> active(nb, 1);
+  auto sort_permute = [&]() {
+    std::vector<PermJob> pj;
+    for (int b = 0; b < nb; ++b) {
+      if (!active[b]) continue;
+      int* p = hp.data() + noff[b];
+      const double* w = hn.data() + noff[b];
+      std::iota(p, p + K[b], 0);
+      std::stable_sort(p, p + K[b], [&](int x, int y) { return w[x] > w[y]; });
+      pj.push_back(PermJob{Wc[b], Wo[b], L[b], K[b], 0, noff[b]});
+      pj.push_back(PermJob{Vc[b], Vo[b], K[b], K[b], 0, noff[b]});
     }
-    const int mp = ncb + (ncb & 1);
-    if ((int)rounds.size() < mp - 1) rounds.resize(mp - 1);
-    for (int r = 0; r < mp - 1; ++r)
-      for (int k = 0; k < mp / 2; ++k) {
-        int P = (k == 0) ? mp - 1 : (r + k) % (mp - 1);
-        int Q = (k == 0) ? r : (r - k + (mp - 1)) % (mp - 1);
-        if (width(P) == 0) std::swap(P, Q);
-        if (width(P) == 0) continue;
-        rounds[r].push_back(PairJob{b, P * CB, width(P), width(Q) ? Q * CB : 0, width(Q), 1});
+    if (pj.empty()) return;
+    SU2_HIP(hipMemcpyAsync(dperm.p, hp.data(), sizeof(int) * ncol, hipMemcpyHostToDevice, st));
+    permute(pj, (const double*)done.p);
+    for (int b = 0; b < nb; ++b) {
+      if (!active[b]) continue;
+      std::swap(Wc[b], Wo[b]);
+      std::swap(Vc[b], Vo[b]);
+    }
+    upload_desc();
+  };
+  // QR preconditioning (precond.hpp) of every block with precondition_min_k
+  // <= K <= kPrecondMaxK: columns sorted, then W <- W P Q2 and V <- P Q2.  A block's
+  // own size decides, so its bits do not depend on the rest of the batch (a
+  // block left out is sorted here instead of at the first sweep: the same
+  // permutation, and the first sweep's sort is then the identity).
+  if (precondition_min_k > 0) {
+    std::vector<PrecondBlock> pb;
+    std::vector<int> pre;
+    for (int b = 0; b < nb; ++b)
+      if (K[b] >= precondition_min_k && K[b] <= kPrecondMaxK) pre.push_back(b);
+    if (!pre.empty()) {
+      sort_permute();
+      for (int b : pre)
+        pb.push_back(PrecondBlock{Wc[b], Vc[b], Wo[b], Vo[b], L[b], K[b], desc[b].normF});
+      precondition_jacobi(c, pb);
+      for (int b : pre) {
+        std::swap(Wc[b], Wo[b]);
+        std::swap(Vc[b], Vo[b]);
       }
+      upload_desc();
+      norms();
+    }
   }
-  // Row slices: a group (the sectors of one wavefunction) with few pairs in
-  // a round spreads each pair over up to MAX_SLICES workgroups (at least 64
-  // rows each), about slice_target workgroups per group and round, so that
-  // the round fills the chip.  The slice count of a pair follows from its own
-  // group alone, never from the other groups of the batch: the slices'
-  // partial Gram matrices are summed in slice order, so the slicing decides
-  // the bits, and a wavefunction gets the same bits alone and in a batch.
-  std::vector<int> round_off(rounds.size() + 1, 0), round_pmax(rounds.size(), 1);
+  // round-robin schedule of column-block pairs of the active blocks, merged
+  // across blocks by round
+  std::vector<std::vector<PairJob>> rounds;
+  std::vector<int> round_off, round_pmax;
   std::vector<PairJob> all;
   size_t gpart_max = 0;
-  for (size_t r = 0; r < rounds.size(); ++r) {
-    round_off[r] = (int)all.size();
-    const int nj = (int)rounds[r].size();
-    std::map<int, int> per_group;
-    for (const PairJob& j : rounds[r]) per_group[blocks[j.blk].group]++;
-    int used = 1;
-    for (PairJob& j : rounds[r]) {
-      const int pmax =
-          std::min(MAX_SLICES, std::max(1, slice_target / per_group[blocks[j.blk].group]));
-      j.P = std::max(1, std::min(pmax, L[j.blk] / MIN_SLICE_ROWS));
-      used = std::max(used, j.P);
+  auto build_schedule = [&]() {
+    rounds.clear();
+    all.clear();
+    gpart_max = 0;
+    for (int b = 0; b < nb; ++b) {
+      if (!active[b]) continue;
+      const int ncb = (K[b] + CB - 1) / CB;
+      auto width = [&](int i) { return i >= ncb ? 0 : std::min(CB, K[b] - i * CB); };
+      if (ncb == 1) {
+        if (rounds.empty()) rounds.resize(1);
+        if (K[b] >= 2) rounds[0].push_back(PairJob{b, 0, K[b], 0, 0, 1});
+        continue;
+      }
+      const int mp = ncb + (ncb & 1);
+      if ((int)rounds.size() < mp - 1) rounds.resize(mp - 1);
+      for (int r = 0; r < mp - 1; ++r)
+        for (int k = 0; k < mp / 2; ++k) {
+          int P = (k == 0) ? mp - 1 : (r + k) % (mp - 1);
+          int Q = (k == 0) ? r : (r - k + (mp - 1)) % (mp - 1);
+          if (width(P) == 0) std::swap(P, Q);
+          if (width(P) == 0) continue;
+          rounds[r].push_back(PairJob{b, P * CB, width(P), width(Q) ? Q * CB : 0, width(Q), 1});
+        }
     }
-    round_pmax[r] = used;
-    gpart_max = std::max(gpart_max, (size_t)nj * used);
-    all.insert(all.end(), rounds[r].begin(), rounds[r].end());
-  }
-  round_off[rounds.size()] = (int)all.size();
-  DevMem dgpart(c, sizeof(double) * NP * NP * std::max<size_t>(1, gpart_max));
+    // Row slices: a group (the sectors of one wavefunction) with few pairs in
+    // a round spreads each pair over up to MAX_SLICES workgroups (at least 64
+    // rows each), about slice_target workgroups per group and round, so that
+    // the round fills the chip.  The slice count of a pair follows from its own
+    // group alone, never from the other groups of the batch: the slices'
+    // partial Gram matrices are summed in slice order, so the slicing decides
+    // the bits, and a wavefunction gets the same bits alone and in a batch.
+    round_off.assign(rounds.size() + 1, 0);
+    round_pmax.assign(rounds.size(), 1);
+    for (size_t r = 0; r < rounds.size(); ++r) {
+      round_off[r] = (int)all.size();
+      const int nj = (int)rounds[r].size();
+      std::map<int, int> per_group;
+      for (const PairJob& j : rounds[r]) per_group[blocks[j.blk].group]++;
+      int used = 1;
+      for (PairJob& j : rounds[r]) {
+        const int pmax =
+            std::min(MAX_SLICES, std::max(1, slice_target / per_group[blocks[j.blk].group]));
+        j.P = std::max(1, std::min(pmax, L[j.blk] / MIN_SLICE_ROWS));
+        used = std::max(used, j.P);
+      }
+      round_pmax[r] = used;
+      gpart_max = std::max(gpart_max, (size_t)nj * used);
+      all.insert(all.end(), rounds[r].begin(), rounds[r].end());
+    }
+    round_off[rounds.size()] = (int)all.size();
+  };
+  // the full schedule sizes the buffers; a smaller one replaces it in place
+  // (a pair of a smaller schedule may take more slices, never more than
+  // MAX_SLICES, and no round has more pairs than the full one's)
+  build_schedule();
+  size_t gpart_cap = 1;
+  for (size_t r = 0; r + 1 < round_off.size(); ++r)
+    gpart_cap = std::max(gpart_cap, (size_t)(round_off[r + 1] - round_off[r]) * MAX_SLICES);
+  DevMem dgpart(c, sizeof(double) * NP * NP * gpart_cap);
   DevMem djobs(c, sizeof(PairJob) * std::max<size_t>(1, all.size())),
       doff(c, sizeof(unsigned long long) * nb);
-  if (!all.empty())
-    SU2_HIP(hipMemcpyAsync(djobs.p, all.data(), sizeof(PairJob) * all.size(),
-                           hipMemcpyHostToDevice, st));
+  auto upload_jobs = [&]() {
+    if (!all.empty())
+      SU2_HIP(hipMemcpyAsync(djobs.p, all.data(), sizeof(PairJob) * all.size(),
+                             hipMemcpyHostToDevice, st));
+  };
+  upload_jobs();
   std::vector<unsigned long long> hoff(nb);
   bool converged = all.empty();
   double worst = 0;
@@ -849,26 +949,7 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
     ++sweeps_done;
     // descending column norms (hn: read back with the previous sweep's
     // convergence counters, or before the first sweep)
-    for (int b = 0; b < nb; ++b) {
-      int* p = hp.data() + noff[b];
-      const double* w = hn.data() + noff[b];
-      std::iota(p, p + K[b], 0);
-      std::stable_sort(p, p + K[b], [&](int x, int y) { return w[x] > w[y]; });
-    }
-    SU2_HIP(hipMemcpyAsync(dperm.p, hp.data(), sizeof(int) * ncol, hipMemcpyHostToDevice, st));
-    {
-      std::vector<PermJob> pj;
-      for (int b = 0; b < nb; ++b) {
-        pj.push_back(PermJob{Wc[b], Wo[b], L[b], K[b], 0, noff[b]});
-        pj.push_back(PermJob{Vc[b], Vo[b], K[b], K[b], 0, noff[b]});
-      }
-      permute(pj, (const double*)done.p);
-    }
-    for (int b = 0; b < nb; ++b) {
-      std::swap(Wc[b], Wo[b]);
-      std::swap(Vc[b], Vo[b]);
-    }
-    upload_desc();
+    sort_permute();
     hipLaunchKernelGGL(fill_u64_kernel, dim3(nblk(nb, 256)), dim3(256), 0, st,
                        (unsigned long long*)doff.p, nb);
     for (size_t r = 0; r < rounds.size(); ++r) {
@@ -896,18 +977,31 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
     // the column norms for the next sweep's order (or the final one) come
     // back with the counters: one wait per sweep
     hipLaunchKernelGGL(colnorm_batched_kernel, dim3((unsigned)ncol), dim3(256), 0, st,
-                       (const BlockDesc*)ddesc.p, (const long long*)dnoff.p, nb, (double*)dn.p);
-    SU2_HIP(hipMemcpyAsync(hn.data(), dn.p, sizeof(double) * ncol, hipMemcpyDeviceToHost, st));
+                       (const BlockDesc*)ddesc.p, (const long long*)dnoff.p, nb, (long long)ncol,
+                       (double*)dn.p);
+    SU2_HIP(hipMemcpyAsync(hn.data(), dn.p, sizeof(double) * 2 * ncol, hipMemcpyDeviceToHost,
+                           st));
     SU2_HIP(hipStreamSynchronize(st));
     converged = true;
     worst = 0;
+    bool left = false;
     for (int b = 0; b < nb; ++b) {
+      if (!active[b]) continue;
       double v;
       std::memcpy(&v, &hoff[b], sizeof v);
       if (v >= 2.0) {   // some pair still rotated this sweep
         converged = false;
         worst = std::max(worst, v - 2.0);
+      } else {
+        active[b] = 0;
+        left = true;
       }
+    }
+    if (left && !converged) {
+      build_schedule();
+      SU2_CHECK(gpart_max <= gpart_cap, kErrCapacity, "jacobi: %zu partial Gram matrices",
+                gpart_max);
+      upload_jobs();
     }
   }
   // Out of sweeps: the columns are accepted when orthogonal to 1e-6 of their
@@ -927,13 +1021,18 @@ int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol,
   for (int b = 0; b < nb; ++b) {
     JacobiBlock& J = blocks[b];
     const int k = K[b];
-    const double* w2 = hn.data() + noff[b];
+    // s_i = |W e_i| / |V e_i|: the singular value of the normalised triplet
+    std::vector<double> s2(k);
+    for (int i = 0; i < k; ++i) {
+      const double v2 = hn[ncol + noff[b] + i];
+      s2[i] = v2 > 0.0 ? hn[noff[b] + i] / v2 : hn[noff[b] + i];
+    }
     int* perm = hp.data() + noff[b];
     std::iota(perm, perm + k, 0);
     std::iota(ident.data() + noff[b], ident.data() + noff[b] + k, 0);
-    std::stable_sort(perm, perm + k, [&](int x, int y) { return w2[x] < w2[y]; });
+    std::stable_sort(perm, perm + k, [&](int x, int y) { return s2[x] < s2[y]; });
     J.s.resize(k);
-    for (int i = 0; i < k; ++i) J.s[i] = std::sqrt(w2[perm[i]]);
+    for (int i = 0; i < k; ++i) J.s[i] = std::sqrt(s2[perm[i]]);
   }
   DevMem dident(c, sizeof(int) * ncol);
   SU2_HIP(hipMemcpyAsync(dperm.p, hp.data(), sizeof(int) * ncol, hipMemcpyHostToDevice, st));

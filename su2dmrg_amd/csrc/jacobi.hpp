@@ -54,8 +54,11 @@ struct JacobiBlock {
 // worst_out (nullable) receives 0 when every block converged, else the
 // largest remaining |w_i.w_j| / |w_i||w_j| (accepted up to 1e-6, beyond it
 // the call throws).
+// precondition_min_k: QR-precondition the blocks with precondition_min_k <=
+// min(m, n) <= kPrecondMaxK first (precond.hpp; 0: none); the results differ
+// from the plain iteration's by rounding only.
 int jacobi_svd_batched(Context& c, std::vector<JacobiBlock>& blocks, double tol = 1e-13,
                        int max_sweeps = 60, int block_cols = 32, int inner_sweeps = 1,
-                       double* worst_out = nullptr);
+                       double* worst_out = nullptr, int precondition_min_k = 0);
 
 }  // namespace su2

@@ -1,0 +1,491 @@
+// Preconditioner of the one-sided Jacobi SVD, see precond.hpp.
+#include "precond.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <type_traits>
+
+#include "common.hpp"
+#include "context.hpp"
+#include "ops.hpp"
+
+namespace su2 {
+
+namespace {
+
+constexpr int PB = 16;        // panel width
+constexpr int PT = 1024;      // threads of the panel kernels
+constexpr int PROWS = 3;      // rows (or columns) per thread of the panel kernels
+static_assert(PT * PROWS == kPrecondMaxK, "panel register tile");
+
+struct PDesc {
+  double* G;   // K x K, column-major: Gram matrix, then the Cholesky factor Rc
+               // (upper, zeros below).  Read row-major it is X = Rc^T (X[i][j]
+               // at G[j + i K]), which the Householder QR then overwrites
+               // with its vectors (below X's diagonal)
+  double* Q;   // K x K, column-major: Q2
+  double* T;   // 256 per panel: the panel's 16 x 16 upper triangular T (column-major)
+  double* Z;   // 16 x K: T-scaled products V^T A of the panel being applied
+  int K;
+  double tiny; // Cholesky pivot floor
+};
+
+// Compile-time loop: f(std::integral_constant<int, k>) for k = K0 .. K1-1, so
+// that register arrays are indexed by constants only (a runtime index would
+// send them to scratch memory).
+template <int K0, int K1, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (K0 < K1) {
+    f(std::integral_constant<int, K0>{});
+    static_for<K0 + 1, K1>(f);
+  }
+}
+
+// Upper Cholesky of G by one 16-row panel: rows c0..c0+bp-1, columns
+// c0..K-1, column j of the row panel on thread j % 1024 (the first two
+// thirds in registers, the last in LDS).  A pivot at or below tiny (or
+// negative from rounding) gives a zero row.  One barrier per step: row k is
+// published before its pivot is used, into alternating LDS buffers.
+__global__ __launch_bounds__(PT) void chol_panel_kernel(const PDesc* __restrict__ desc, int p) {
+  const PDesc D = desc[blockIdx.x];
+  const int K = D.K, c0 = PB * p;
+  if (c0 >= K) return;
+  const int bp = min(PB, K - c0), nc = K - c0;
+  const int t = threadIdx.x;
+  __shared__ double lw[PT * (PB + 1)];   // third register column, padded rows
+  __shared__ double s_raw[2][PB];
+  double w[2][PB];
+  double* gcol[PROWS];
+#pragma unroll
+  for (int q = 0; q < PROWS; ++q) {
+    const int j = t + PT * q;
+    gcol[q] = D.G + (size_t)(c0 + min(j, nc - 1)) * K + c0;
+  }
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int j = t + PT * q;
+#pragma unroll
+    for (int i = 0; i < PB; ++i) w[q][i] = (j < nc && i < bp) ? gcol[q][i] : 0.0;
+  }
+  double* l2 = lw + t * (PB + 1);
+  const int j2 = t + 2 * PT;
+#pragma unroll
+  for (int i = 0; i < PB; ++i) l2[i] = (j2 < nc && i < bp) ? gcol[2][i] : 0.0;
+  static_for<0, PB>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    __builtin_amdgcn_sched_barrier(0);
+    if (k < bp) {
+      double* raw = s_raw[k & 1];
+      // row k as it stands (columns c0 + k .. c0 + 15 live on threads k..15)
+      if (t >= k && t < PB) raw[t] = w[0][k];
+      __syncthreads();
+      const double dk = raw[k];
+      const double inv = dk > D.tiny ? 1.0 / sqrt(dk) : 0.0;
+      // R[k][j] for my columns (zero left of the diagonal), then the rows
+      // below: w[i][j] -= R[k][i] R[k][j] for k < i <= j
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int j = t + PT * q;
+        const double rkj = (j >= k) ? w[q][k] * inv : 0.0;
+        w[q][k] = rkj;
+#pragma unroll
+        for (int i = k + 1; i < PB; ++i)
+          if (i < bp && i <= j) w[q][i] = fma(-raw[i] * inv, rkj, w[q][i]);
+      }
+      {
+        const double rkj = l2[k] * inv;   // j2 >= 2048 > k
+        l2[k] = rkj;
+#pragma unroll
+        for (int i = k + 1; i < PB; ++i)
+          if (i < bp) l2[i] = fma(-raw[i] * inv, rkj, l2[i]);
+      }
+    }
+  });
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int j = t + PT * q;
+    if (j >= nc) continue;
+#pragma unroll
+    for (int i = 0; i < PB; ++i)
+      if (i < bp) gcol[q][i] = (i > j) ? 0.0 : w[q][i];
+  }
+  if (j2 < nc) {
+#pragma unroll
+    for (int i = 0; i < PB; ++i)
+      if (i < bp) gcol[2][i] = l2[i];
+  }
+  // the panel's columns below its diagonal block: zero (Rc is upper
+  // triangular, and read row-major they are X's entries above its diagonal)
+  for (int e = t; e < bp * (nc - bp); e += PT) {
+    const int i = e / (nc - bp), r = e % (nc - bp);
+    D.G[(size_t)(c0 + bp + r) + (size_t)(c0 + i) * K] = 0.0;
+  }
+}
+
+// Trailing update of the Cholesky: G[i][j] -= sum_k R[c0+k][i] R[c0+k][j]
+// for c1 <= i <= j (upper triangle only; the panel kernels read nothing
+// below it).  64 x 64 tiles, k summed in order.
+__global__ __launch_bounds__(256) void chol_trail_kernel(const PDesc* __restrict__ desc, int p) {
+  const PDesc D = desc[blockIdx.z];
+  const int K = D.K, c0 = PB * p, c1 = c0 + PB;
+  if (c1 >= K) return;
+  const int ti = blockIdx.x, tj = blockIdx.y;
+  const int n = K - c1;
+  if (ti > tj || ti * 64 >= n || tj * 64 >= n) return;
+  __shared__ double Ri[PB][64], Rj[PB][64];
+  const int t = threadIdx.x;
+  for (int e = t; e < PB * 64; e += 256) {
+    const int k = e / 64, x = e % 64;
+    const int gi = c1 + ti * 64 + x, gj = c1 + tj * 64 + x;
+    Ri[k][x] = gi < K ? D.G[(size_t)(c0 + k) + (size_t)gi * K] : 0.0;
+    Rj[k][x] = gj < K ? D.G[(size_t)(c0 + k) + (size_t)gj * K] : 0.0;
+  }
+  __syncthreads();
+  const int x = t % 64;                 // row within the tile (contiguous in memory)
+  const int gi = c1 + ti * 64 + x;
+  for (int s = 0; s < 16; ++s) {
+    const int y = t / 64 + 4 * s;       // column within the tile
+    const int gj = c1 + tj * 64 + y;
+    if (gi < K && gj < K && gi <= gj) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < PB; ++k) acc = fma(Ri[k][x], Rj[k][y], acc);
+      D.G[(size_t)gi + (size_t)gj * K] -= acc;
+    }
+  }
+}
+
+// Householder QR of one 16-column panel of X (rows c0..K-1; row c0 + r on
+// thread r % 1024, the first two thirds in registers, the last in LDS).
+// Column j becomes its Householder vector below the diagonal (the unit
+// diagonal implicit); the panel's T factor of the compact WY form H_0 ...
+// H_{b-1} = I - V T V^T goes to D.T.  Each step takes one block reduction of
+// the 16 products d_c = sum_{r>j} x_j(r) x_c(r): with row j, they give
+// sigma^2 = d_j, w_c = v_j^T x_c = x_c(j) + scale d_c (c > j) and
+// y_c = v_c^T v_j = x_c(j) + scale d_c (c < j, for T).
+__global__ __launch_bounds__(PT) void hh_panel_kernel(const PDesc* __restrict__ desc, int p) {
+  const PDesc D = desc[blockIdx.x];
+  const int K = D.K, c0 = PB * p;
+  if (c0 >= K) return;
+  const int bp = min(PB, K - c0), m = K - c0;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  __shared__ double lx[PT * (PB + 1)];   // third register row, padded
+  __shared__ double s_red[2][PT / 64][PB + 1];
+  __shared__ double s_row[2][PB];
+  __shared__ double s_d[2][PB];
+  __shared__ double s_T[PB][PB];
+  double x[2][PB];
+  double* xrow[PROWS];
+#pragma unroll
+  for (int s = 0; s < PROWS; ++s) {
+    const int r = t + PT * s;
+    xrow[s] = D.G + (size_t)(c0 + min(r, m - 1)) * K + c0;
+  }
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int r = t + PT * s;
+#pragma unroll
+    for (int c = 0; c < PB; ++c) x[s][c] = (r < m && c < bp) ? xrow[s][c] : 0.0;
+  }
+  double* l2 = lx + t * (PB + 1);
+  const int r2 = t + 2 * PT;
+#pragma unroll
+  for (int c = 0; c < PB; ++c) l2[c] = (r2 < m && c < bp) ? xrow[2][c] : 0.0;
+  if (t < PB * PB) s_T[t % PB][t / PB] = 0.0;
+  static_for<0, PB>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    __builtin_amdgcn_sched_barrier(0);
+    if (j < bp) {
+      const int b = j & 1;
+      // d_c over my rows below the diagonal, summed over each wave
+#pragma unroll
+      for (int c = 0; c < PB; ++c) {
+        double v = 0.0;
+        if (c < bp) {
+#pragma unroll
+          for (int s = 0; s < 2; ++s) {
+            const int r = t + PT * s;
+            if (r > j && r < m) v = fma(x[s][j], x[s][c], v);
+          }
+          if (r2 > j && r2 < m) v = fma(l2[j], l2[c], v);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) s_red[b][wave][c] = v;
+      }
+      if (t == j) {
+#pragma unroll
+        for (int c = 0; c < PB; ++c) s_row[b][c] = x[0][c];
+      }
+      __syncthreads();
+      if (t < PB) {
+        double v = 0.0;
+#pragma unroll
+        for (int w = 0; w < PT / 64; ++w) v += s_red[b][w][t];
+        s_d[b][t] = v;
+      }
+      __syncthreads();
+      // the reflector, computed alike on every thread
+      const double alpha = s_row[b][j], sig2 = s_d[b][j];
+      double tau = 0.0, scale = 0.0;
+      if (sig2 != 0.0) {
+        const double beta = -copysign(sqrt(fma(alpha, alpha, sig2)), alpha);
+        tau = (beta - alpha) / beta;
+        scale = 1.0 / (alpha - beta);
+      }
+      // T column j (T[t][k], k < j, was written by this same thread)
+      if (t < j) {
+        double acc = 0.0;
+        for (int k = t; k < j; ++k) acc = fma(s_T[t][k], fma(scale, s_d[b][k], s_row[b][k]), acc);
+        s_T[t][j] = -tau * acc;
+      }
+      if (t == j) s_T[j][j] = tau;
+      // v_j into column j, H_j applied to the later columns (rows >= j)
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int r = t + PT * s;
+        if (r >= j && r < m) {
+          const double vr = (r == j) ? 1.0 : x[s][j] * scale;
+          if (r > j) x[s][j] = vr;
+#pragma unroll
+          for (int c = j + 1; c < PB; ++c)
+            if (c < bp) x[s][c] = fma(-tau * vr, fma(scale, s_d[b][c], s_row[b][c]), x[s][c]);
+        }
+      }
+      if (r2 > j && r2 < m) {
+        const double vr = l2[j] * scale;
+        l2[j] = vr;
+#pragma unroll
+        for (int c = j + 1; c < PB; ++c)
+          if (c < bp) l2[c] = fma(-tau * vr, fma(scale, s_d[b][c], s_row[b][c]), l2[c]);
+      }
+    }
+  });
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int r = t + PT * s;
+    if (r >= m) continue;
+#pragma unroll
+    for (int c = 0; c < PB; ++c)
+      if (c < bp) xrow[s][c] = x[s][c];
+  }
+  if (r2 < m) {
+#pragma unroll
+    for (int c = 0; c < PB; ++c)
+      if (c < bp) xrow[2][c] = l2[c];
+  }
+  if (t < PB * PB) D.T[(size_t)p * PB * PB + t] = s_T[t % PB][t / PB];
+}
+
+// Householder vector i of panel p at panel row r (the unit diagonal implicit).
+__device__ __forceinline__ double hh_v(const PDesc& D, int c0, int bp, int m, int i, int r) {
+  if (i >= bp || r >= m || r < i) return 0.0;
+  if (r == i) return 1.0;
+  return D.G[(size_t)(c0 + r) * D.K + (c0 + i)];
+}
+
+// Z = op(T) (V^T A) for 16 columns of A (rows c0..K-1): A = X from column
+// c0 + bp (the QR's trailing columns, op(T) = T^T) or A = Q from column c0
+// (Q2's backward formation, op(T) = T).  1024 threads: four row groups of
+// 256 threads, each a 16 x 16 block of sums over its 64-row chunks (through
+// LDS, in order), then the four groups summed in order.
+__global__ __launch_bounds__(1024) void hh_dots_kernel(const PDesc* __restrict__ desc, int p,
+                                                       int qr) {
+  const PDesc D = desc[blockIdx.y];
+  const int K = D.K, c0 = PB * p;
+  if (c0 >= K) return;
+  const int bp = min(PB, K - c0), m = K - c0;
+  const int colstart = c0 + (qr ? bp : 0), ncols = K - colstart;
+  const int chunk = blockIdx.x * PB;
+  if (chunk >= ncols) return;
+  // A(row, col): X row-major (the QR's trailing columns) or Q column-major
+  const double* A = qr ? D.G : D.Q;
+  const size_t rs = qr ? (size_t)K : 1, cs = qr ? 1 : (size_t)K;
+  constexpr int NG = 4;
+  __shared__ double sv[NG][64][PB + 1], sa[NG][64][PB + 1];
+  __shared__ double sd[NG][PB][PB + 1];
+  const int t = threadIdx.x, g = t / 256, tg = t % 256;
+  const int i = tg / PB, c = tg % PB;
+  double acc = 0.0;
+  // every thread runs the same iterations (the barriers inside need that);
+  // a group past the last row loads zeros
+  for (int base = 0; base < m; base += 64 * NG) {
+    const int r0 = base + 64 * g;
+    for (int e = tg; e < 64 * PB; e += 256) {
+      // X: the 16 columns of a row are contiguous, so k runs fastest
+      const int rr = e / PB, k = e % PB;
+      const int r = r0 + rr;
+      sv[g][rr][k] = hh_v(D, c0, bp, m, k, r);
+      if (qr) {
+        const int col = chunk + k;
+        sa[g][rr][k] = (r < m && col < ncols) ? A[(size_t)(c0 + r) * rs + (size_t)(colstart + col) * cs] : 0.0;
+      }
+    }
+    if (!qr) {
+      for (int e = tg; e < 64 * PB; e += 256) {
+        // Q: the rows of a column are contiguous, so rr runs fastest
+        const int rr = e % 64, k = e / 64;
+        const int r = r0 + rr, col = chunk + k;
+        sa[g][rr][k] = (r < m && col < ncols) ? A[(size_t)(c0 + r) * rs + (size_t)(colstart + col) * cs] : 0.0;
+      }
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int rr = 0; rr < 64; ++rr) acc = fma(sv[g][rr][i], sa[g][rr][c], acc);
+    __syncthreads();
+  }
+  sd[g][i][c] = acc;
+  __syncthreads();
+  if (g == 0) {
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < NG; ++q) v += sd[q][i][c];
+    sd[0][i][c] = v;
+  }
+  __syncthreads();
+  if (g == 0) {
+    const double* T = D.T + (size_t)p * PB * PB;   // T[a][b] at a + 16 b
+    double z = 0.0;
+#pragma unroll
+    for (int k = 0; k < PB; ++k) z = fma(qr ? T[k + PB * i] : T[i + PB * k], sd[0][k][c], z);
+    if (chunk + c < ncols) D.Z[(size_t)i + (size_t)PB * (chunk + c)] = z;
+  }
+}
+
+// A -= V Z on rows c0..K-1 of the same columns as hh_dots_kernel; 64 x 16
+// tiles, the 16-term sums in order.
+__global__ __launch_bounds__(256) void hh_update_kernel(const PDesc* __restrict__ desc, int p,
+                                                        int qr) {
+  const PDesc D = desc[blockIdx.z];
+  const int K = D.K, c0 = PB * p;
+  if (c0 >= K) return;
+  const int bp = min(PB, K - c0), m = K - c0;
+  const int colstart = c0 + (qr ? bp : 0), ncols = K - colstart;
+  const int chunk = blockIdx.x * PB, r0 = blockIdx.y * 64;
+  if (chunk >= ncols || r0 >= m) return;
+  double* A = qr ? D.G : D.Q;
+  const size_t rs = qr ? (size_t)K : 1, cs = qr ? 1 : (size_t)K;
+  __shared__ double sv[64][PB + 1];
+  __shared__ double sz[PB][PB + 1];
+  const int t = threadIdx.x;
+  for (int e = t; e < 64 * PB; e += 256) {
+    const int rr = e / PB, k = e % PB;
+    sv[rr][k] = hh_v(D, c0, bp, m, k, r0 + rr);
+  }
+  {
+    const int k = t % PB, cc = t / PB;
+    sz[k][cc] = (chunk + cc < ncols) ? D.Z[(size_t)k + (size_t)PB * (chunk + cc)] : 0.0;
+  }
+  __syncthreads();
+  // X: a thread per column of a row (contiguous); Q: a thread per row of a
+  // column (contiguous)
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int rr = qr ? t / PB + 16 * s : t % 64;
+    const int cc = qr ? t % PB : t / 64 + 4 * s;
+    const int r = r0 + rr;
+    if (r >= m || chunk + cc >= ncols) continue;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < PB; ++k) acc = fma(sv[rr][k], sz[k][cc], acc);
+    A[(size_t)(c0 + r) * rs + (size_t)(colstart + chunk + cc) * cs] -= acc;
+  }
+}
+
+__global__ void eye_batched_kernel(const PDesc* __restrict__ desc) {
+  const PDesc D = desc[blockIdx.y];
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = (long long)D.K * D.K;
+  if (e < n) D.Q[e] = (e % D.K) == (e / D.K) ? 1.0 : 0.0;
+}
+
+unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace
+
+void precondition_jacobi(Context& c, std::vector<PrecondBlock>& blocks) {
+  const int nb = (int)blocks.size();
+  if (nb == 0) return;
+  SU2_CHECK(nb < 65536, kErrCapacity, "jacobi preconditioner: %d blocks", nb);
+  hipStream_t st = c.stream();
+  int kmax = 0;
+  long long scratch = 0;
+  for (auto& b : blocks) {
+    SU2_CHECK(b.K >= 1 && b.K <= kPrecondMaxK && b.L >= b.K, kErrArgument,
+              "jacobi preconditioner: block %d x %d", b.L, b.K);
+    kmax = std::max(kmax, b.K);
+    const long long np = (b.K + PB - 1) / PB;
+    scratch += 2LL * b.K * b.K + np * PB * PB + (long long)PB * b.K;
+  }
+  DevBuf work(c, scratch);
+  std::vector<PDesc> hd(nb);
+  long long off = 0;
+  for (int i = 0; i < nb; ++i) {
+    const PrecondBlock& b = blocks[i];
+    const long long kk = (long long)b.K * b.K, np = (b.K + PB - 1) / PB;
+    PDesc& d = hd[i];
+    d.G = work.p + off;
+    d.Q = d.G + kk;
+    d.T = d.Q + kk;
+    d.Z = d.T + np * PB * PB;
+    off += 2 * kk + np * PB * PB + (long long)PB * b.K;
+    d.K = b.K;
+    d.tiny = 1e-15 * b.normF * b.normF;
+  }
+  DevBuf ddesc(c, (long long)((sizeof(PDesc) * nb + 7) / 8));
+  SU2_HIP(hipMemcpyAsync(ddesc.p, hd.data(), sizeof(PDesc) * nb, hipMemcpyHostToDevice, st));
+  const PDesc* dd = (const PDesc*)ddesc.p;
+  // Gram matrices
+  {
+    GemmBatch g;
+    for (int i = 0; i < nb; ++i) {
+      const PrecondBlock& b = blocks[i];
+      g.add(hd[i].G, b.K, b.K, b.K, b.W, b.L, 1, b.W, b.L, 0, b.L);
+    }
+    g.run(c);
+  }
+  const int npmax = (kmax + PB - 1) / PB;
+  // Cholesky by row panels
+  for (int p = 0; p < npmax; ++p) {
+    hipLaunchKernelGGL(chol_panel_kernel, dim3(nb), dim3(PT), 0, st, dd, p);
+    const int n = kmax - PB * (p + 1);
+    if (n > 0) {
+      const unsigned nt = cdiv(n, 64);
+      hipLaunchKernelGGL(chol_trail_kernel, dim3(nt, nt, nb), dim3(256), 0, st, dd, p);
+    }
+  }
+  // Householder QR of Rc^T by column panels (trailing columns updated)
+  for (int p = 0; p < npmax; ++p) {
+    hipLaunchKernelGGL(hh_panel_kernel, dim3(nb), dim3(PT), 0, st, dd, p);
+    const int n = kmax - PB * (p + 1);
+    if (n > 0) {
+      hipLaunchKernelGGL(hh_dots_kernel, dim3(cdiv(n, PB), nb), dim3(1024), 0, st, dd, p, 1);
+      hipLaunchKernelGGL(hh_update_kernel, dim3(cdiv(n, PB), cdiv(kmax - PB * p, 64), nb),
+                         dim3(256), 0, st, dd, p, 1);
+    }
+  }
+  // Q2 = H_0 ... H_{np-1}, formed backwards from the identity
+  hipLaunchKernelGGL(eye_batched_kernel, dim3(cdiv((long long)kmax * kmax, 256), nb), dim3(256),
+                     0, st, dd);
+  for (int p = npmax - 1; p >= 0; --p) {
+    const int n = kmax - PB * p;
+    hipLaunchKernelGGL(hh_dots_kernel, dim3(cdiv(n, PB), nb), dim3(1024), 0, st, dd, p, 0);
+    hipLaunchKernelGGL(hh_update_kernel, dim3(cdiv(n, PB), cdiv(n, 64), nb), dim3(256), 0, st, dd,
+                       p, 0);
+  }
+  SU2_HIP(hipGetLastError());
+  // W Q2 and V Q2
+  {
+    GemmBatch g;
+    for (int i = 0; i < nb; ++i) {
+      const PrecondBlock& b = blocks[i];
+      g.add(b.Wout, b.L, b.L, b.K, b.W, b.L, 0, hd[i].Q, b.K, 0, b.K);
+      g.add(b.Vout, b.K, b.K, b.K, b.V, b.K, 0, hd[i].Q, b.K, 0, b.K);
+    }
+    g.run(c);
+  }
+}
+
+}  // namespace su2

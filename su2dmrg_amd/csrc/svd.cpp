@@ -188,7 +188,7 @@ std::vector<SvdResult> truncated_svd_batch(Context& c, const std::vector<SvdJob>
     }
     const auto tj = std::chrono::steady_clock::now();
     jacobi_sweeps = jacobi_svd_batched(c, jb, 1e-13, 60, c.svd_block_cols, c.svd_inner_sweeps,
-                                       &jacobi_worst);
+                                       &jacobi_worst, c.svd_precondition_min_k);
     t_jacobi = std::chrono::duration<double>(std::chrono::steady_clock::now() - tj).count();
     for (size_t t = 0; t < jb.size(); ++t) {
       std::vector<double>& w = wv[idx[t].first][idx[t].second];

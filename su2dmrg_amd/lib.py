@@ -38,6 +38,7 @@ _SIG = {
     "su2_last_error": (C.c_char_p, [_p]),
     "su2_ctx_synchronize": (_i, [_p]),
     "su2_ctx_set_svd_tuning": (_i, [_p, _i, _i]),
+    "su2_ctx_set_svd_precondition": (_i, [_p, _i]),
     "su2_ctx_memory": (_i, [_p, _llp, _llp, _llp, _i]),
     "su2_tensor_create": (_i, [_p, _i, _ip, _ip, _ip, _ip, _i, _ip, C.POINTER(_p)]),
     "su2_tensor_destroy": (_i, [_p, _p]),
@@ -168,6 +169,14 @@ class Context:
 
     def sync(self):
         self.check(lib.su2_ctx_synchronize(self.h))
+
+    def set_svd_precondition(self, min_k=512):
+        """QR preconditioning of the Jacobi SVD for sector blocks with
+        min(m, n) >= min_k (0 or False: off, the context's default; True:
+        512)."""
+        if min_k is True:
+            min_k = 512
+        self.check(lib.su2_ctx_set_svd_precondition(self.h, int(min_k)))
 
     def set_svd_tuning(self, block_cols=32, inner_sweeps=1):
         """Column block width (16 or 32) and inner sweeps per pair visit (1..3)

@@ -68,7 +68,8 @@ def test_truncated_svd_matches_oracle(ctx, p1, p2, dcut):
 
 
 @pytest.mark.parametrize("block,inner", [(32, 1), (32, 3), (16, 1)])
-@pytest.mark.parametrize("m,n,decades", [(300, 200, 20), (180, 420, 12), (520, 500, 6)])
+@pytest.mark.parametrize("m,n,decades", [(300, 200, 20), (180, 420, 12), (520, 500, 6),
+                                          (700, 560, 14), (600, 1100, 8)])
 def test_svd_wide_spectrum_against_lapack(ctx, m, n, decades, block, inner):
     # Singular values spread over many decades, as in a converged DMRG
     # wavefunction: every value to 1e-12 of the largest (the Gram route of the
@@ -98,6 +99,35 @@ def test_svd_wide_spectrum_against_lapack(ctx, m, n, decades, block, inner):
     sb = s[big]
     bound = 1e-13 * s[-1] / np.minimum.outer(sb, sb) + 1e-13
     assert np.all(np.abs(Ub.T @ Ub - np.eye(big.sum())) <= bound)
+
+
+@pytest.mark.parametrize("m,n,decades", [(700, 560, 14), (600, 1100, 8), (1300, 1290, 16)])
+def test_svd_preconditioned_equals_plain(ctx, m, n, decades):
+    # QR preconditioning (here of every block from min(m, n) = 48) changes
+    # where the Jacobi iteration starts, not its accuracy: both runs match
+    # LAPACK, and each other, to 1e-12 of the largest value
+    from su2dmrg_amd import lib as L
+    _say(f"svd {m}x{n}, {decades} decades, preconditioned and plain")
+    rng = np.random.default_rng(m * n)
+    k = min(m, n)
+    u, _ = np.linalg.qr(rng.standard_normal((m, k)))
+    v, _ = np.linalg.qr(rng.standard_normal((n, k)))
+    A = (u * np.logspace(0, -decades, k)) @ v.T
+    ref = np.sort(np.linalg.svd(A, compute_uv=False))
+    out = {}
+    for pre in (True, False):
+        ctx.set_svd_precondition(48 if pre else 0)
+        t = L.wavefunction(ctx, [1], [m], [n], {1: A})
+        out[pre] = L.svd_truncate(ctx, t, 0, 1, 100000)
+    ctx.set_svd_precondition(0)
+    for pre in (True, False):
+        U, V, kept, _ = out[pre]
+        s = np.array(kept)
+        np.testing.assert_allclose(s, ref, rtol=0, atol=1e-12 * ref[-1])
+        cut = s[s <= 1e-11 * s[-1]].sum()
+        np.testing.assert_allclose(U[1] @ V[1].T, A, rtol=0, atol=1e-13 * ref[-1] + cut)
+    np.testing.assert_allclose(np.array(out[True][2]), np.array(out[False][2]), rtol=0,
+                               atol=1e-12 * ref[-1])
 
 
 def _run_both(ctx, lx, ly, jcoup, bdry, D, nsweep, totspin=0):
