@@ -303,10 +303,13 @@ def _gram_eig(G):
     n = G.shape[0]
     w, v = np.linalg.eigh(G)
     npp = 0
-    for i in range(n - 1, -1, -1):
-        if w[i] / w[n - 1] < 0.1:
-            npp = i + 1
-            break
+    # a zero Gram matrix gives 0/0: NaN compares false, as in the reference's
+    # C loop, and no pair is refined
+    with np.errstate(invalid="ignore", divide="ignore"):
+        for i in range(n - 1, -1, -1):
+            if w[i] / w[n - 1] < 0.1:
+                npp = i + 1
+                break
     if npp > 1:
         ap = G.copy()
         vp = v.copy()
