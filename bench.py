@@ -27,8 +27,10 @@ legs of the same line (each can be switched off):
 * serve: a Poisson stream of mixed jobs (lattices, couplings, bond
   dimensions, ground and excited levels) through the continuous-batching
   server with 16 slots over --lanes lanes, jobs/hour and p50/p99 latency
-  against the same jobs served one at a time and with the 16 slots in one
-  lane;
+  against the same jobs served one at a time (and, with --serve-one-lane,
+  with the 16 slots in one lane); 64 jobs by default, so that the start of
+  the stream and the drain of its last jobs are a small part of the wall
+  time;
 * cpu_sweep: the oracle's CPU sweep of a 16x8 cylinder at a small D beside
   the device's sweep of the same lattice and D.
 """
@@ -287,7 +289,7 @@ def scan_leg(L, ctx, njobs, nsweep, dvals, lanes=1):
     return out
 
 
-def serve_leg(L, ctx, njobs, rate_per_s, lattices, dvals, seed=3, lanes=1):
+def serve_leg(L, ctx, njobs, rate_per_s, lattices, dvals, seed=3, lanes=1, one_lane=False):
     import numpy as np
     rng = np.random.default_rng(seed)
     jobs = []
@@ -323,7 +325,7 @@ def serve_leg(L, ctx, njobs, rate_per_s, lattices, dvals, seed=3, lanes=1):
            "excited_share": 0.5, "slots": 16, "lanes": lanes, "budget_bytes": budget}
     out.update(main)
     out["sequential"] = seq
-    if lanes > 1:
+    if lanes > 1 and one_lane:
         resb, one = serve(16, 1)            # the 16 slots in one lane
         out["one_lane"] = one
         out["one_lane"]["throughput_ratio"] = round(seq["wall_s"] / one["wall_s"], 3)
@@ -383,7 +385,9 @@ def main():
     ap.add_argument("--scan-jobs", type=int, default=32)
     ap.add_argument("--scan-sweeps", type=int, default=1)
     ap.add_argument("--scan-d", default="300,500", help="bond dimensions drawn by the scan jobs")
-    ap.add_argument("--serve-jobs", type=int, default=32)
+    ap.add_argument("--serve-jobs", type=int, default=64)
+    ap.add_argument("--serve-one-lane", action="store_true",
+                    help="also serve the stream with the 16 slots in one lane")
     ap.add_argument("--serve-rate", type=float, default=8.0)
     ap.add_argument("--serve-lattices", default="6x4,8x4,8x6,12x6")
     ap.add_argument("--serve-d", default="64,100,150")
@@ -479,11 +483,13 @@ def main():
         log(f"serve: {args.serve_jobs} jobs at {args.serve_rate}/s")
         serve = serve_leg(L, ctx, args.serve_jobs, args.serve_rate,
                           [tuple(int(y) for y in x.split("x")) for x in args.serve_lattices.split(",")],
-                          [int(x) for x in args.serve_d.split(",")], lanes=args.lanes)
+                          [int(x) for x in args.serve_d.split(",")], lanes=args.lanes,
+                          one_lane=args.serve_one_lane)
         log(f"serve: {serve['jobs_per_hour']} jobs/h ({args.lanes} lanes), p50 "
             f"{serve['p50_latency_s']} s, p99 {serve['p99_latency_s']} s; sequential "
             f"{serve['sequential']['jobs_per_hour']} jobs/h"
-            + (f"; one lane {serve['one_lane']['jobs_per_hour']} jobs/h" if args.lanes > 1 else ""))
+            + (f"; one lane {serve['one_lane']['jobs_per_hour']} jobs/h" if "one_lane" in serve else "")
+            + f"; ratio {serve['throughput_ratio']}")
 
     cpu = None
     cpu_sweep = None

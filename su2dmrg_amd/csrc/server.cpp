@@ -58,6 +58,7 @@ int Server::submit(const JobSpec& s, double arrival_s) {
 }
 
 struct Server::Lane {
+  int idx = 0;
   Context* c = nullptr;
   std::unique_ptr<Context> own;   // extra lanes' contexts
   std::unique_ptr<BatchDriver> drv;
@@ -107,6 +108,11 @@ void Server::lane_loop(Lane& l) {
         auto it = std::find_if(l.slots.begin(), l.slots.end(),
                                [](const Slot& s) { return s.job < 0; });
         if (it == l.slots.end() || used_ + res_[id].footprint > budget_) break;
+        // the job goes to a lane with a free slot and the fewest live jobs
+        int least = lane_live_[l.idx];
+        for (size_t k = 0; k < lane_live_.size(); ++k)
+          if (lane_live_[k] < lane_slots_[k]) least = std::min(least, lane_live_[k]);
+        if (lane_live_[l.idx] > least) break;
         ++next_;
         Slot& s = *it;
         s.job = id;
@@ -117,6 +123,7 @@ void Server::lane_loop(Lane& l) {
         used_ += res_[id].footprint;
         res_[id].t_admit = t;
         ++live_;
+        ++lane_live_[l.idx];
         max_conc_ = std::max(max_conc_, live_);
         admitted.push_back(&s);
       }
@@ -161,6 +168,7 @@ void Server::lane_loop(Lane& l) {
         used_ -= r.footprint;
         ++finished_;
         --live_;
+        --lane_live_[l.idx];
       }
       if (verbose_)
         std::fprintf(stderr, "server: t=%.2fs job %d done, E=%.12f\n", r.t_finish, s.job,
@@ -186,8 +194,11 @@ void Server::run() {
   max_conc_ = 0;
   const int nl = std::min(lanes_, max_slots_);
   std::vector<std::unique_ptr<Lane>> lanes;
+  lane_live_.assign(nl, 0);
+  lane_slots_.assign(nl, 0);
   for (int k = 0; k < nl; ++k) {
     std::unique_ptr<Lane> l(new Lane());
+    l->idx = k;
     if (k == 0) {
       l->c = &c_;
     } else {
@@ -196,6 +207,7 @@ void Server::run() {
     }
     l->drv.reset(new BatchDriver(*l->c));
     l->slots.resize(max_slots_ / nl + (k < max_slots_ % nl ? 1 : 0));
+    lane_slots_[k] = (int)l->slots.size();
     lanes.push_back(std::move(l));
   }
   std::vector<std::exception_ptr> err(nl);

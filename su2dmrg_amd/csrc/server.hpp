@@ -13,9 +13,11 @@
 //   * a finished job frees its slot and memory at once; a job arriving while
 //     others run joins at the next tick;
 //   * with several lanes, the slots are split over lanes, each with its own
-//     context (stream), BatchDriver and host thread; a lane takes the next
-//     arrived job that fits when one of its slots is free, and the lanes'
-//     host work overlaps each other's device work (run_batches_parallel).
+//     context (stream), BatchDriver and host thread; the next arrived job
+//     that fits goes to a lane with a free slot and the fewest live jobs
+//     (a lane's tick waits for its slowest job's step, so spreading the jobs
+//     shortens every tick), and the lanes' host work overlaps each other's
+//     device work (run_batches_parallel).
 // An excited-state job (level k > 0) runs its ground state and the levels
 // below k first, each stage saving its state in the reference's file format
 // for the next (add_lower_state), all inside its slot.
@@ -99,6 +101,7 @@ class Server {
   // shared by the lanes of one run(), under mu_
   std::mutex mu_;
   std::vector<int> order_;
+  std::vector<int> lane_live_, lane_slots_;   // live jobs and slots per lane
   size_t next_ = 0;
   double used_ = 0, t0_ = 0;
   int finished_ = 0, live_ = 0;
