@@ -133,10 +133,11 @@ def centre_matvec(L, ctx, dm, reps=10):
     return rec
 
 
-def run_schedule(L, ctx, lx, ly, schedule, label, centre=False):
+def run_schedule(L, ctx, lx, ly, schedule, label, centre=False, graph=True):
     """Warm-up then one sweep per (D, bits) entry; per-sweep wall times (device
     synchronised on both sides), energies and the device-side breakdown."""
     dm = L.Dmrg(ctx, lx, ly, 2.0, 0.0, 2, 0, schedule[0][0])
+    dm.set_graph(graph)
     d0, b0 = schedule[0]
     dm.set_max_dcut(d0)
     dm.set_matvec_precision(b0)
@@ -191,11 +192,12 @@ def sweep_legs(L, ctx, args):
     lx, ly = (int(x) for x in args.sweep_lattice.split("x"))
     D = args.sweep_d
     n = args.sweeps
-    fp64 = run_schedule(L, ctx, lx, ly, [(D, 64)] * n, "fp64 run", centre=True)
+    g = not args.no_graph
+    fp64 = run_schedule(L, ctx, lx, ly, [(D, 64)] * n, "fp64 run", centre=True, graph=g)
     ramp = [d for d in (500, 1000, 2000) if d < D] + [D]
     ramp = (ramp + [D] * 3)[:max(1, n - 2)]
     mixed_sched = [(d, 32) for d in ramp] + [(D, 64)] * min(2, n)
-    mixed = run_schedule(L, ctx, lx, ly, mixed_sched, "mixed run")
+    mixed = run_schedule(L, ctx, lx, ly, mixed_sched, "mixed run", graph=g)
     d_e = abs(mixed["final_energy_per_site"] - fp64["final_energy_per_site"])
     return {"fp64": fp64, "mixed": mixed, "mixed_vs_fp64_abs_de_per_site": d_e,
             "note": "both runs start from the reference's all-ones warm-up vector; with "
@@ -392,6 +394,9 @@ def main():
     ap.add_argument("--serve-lattices", default="6x4,8x4,8x6,12x6")
     ap.add_argument("--serve-d", default="64,100,150")
     ap.add_argument("--cpu-sweep", default="16x8:64")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="sweep legs: launch each Lanczos matvec directly instead of replaying "
+                         "its HIP graph (same kernels and bits; for kernel tracers)")
     ap.add_argument("--svd-block", type=int, default=32, choices=[16, 32])
     ap.add_argument("--svd-inner", type=int, default=1, choices=[1, 2, 3])
     args = ap.parse_args()
