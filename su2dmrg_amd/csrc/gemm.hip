@@ -304,14 +304,17 @@ __device__ __forceinline__ void store_slab(const GemmSlice& s,
 // registers per lane; one wave's MFMAs run while the other loads, stores its
 // next slab or waits at the barrier), one for larger sub-tiles, whose
 // accumulators take the 256 accumulation registers and whose MFMA work per
-// slab covers the barrier by itself.
-template <int BM, int BN, int WM, int WN, int ES>
+// slab covers the barrier by itself, and one where the two LDS slab buffers
+// take more than half of the CU's 160 KB (a second workgroup cannot start).
+template <int BM, int BN, int BK, int WM, int WN, int PAD, int ES>
 struct WavesPerSimd {
-  static constexpr int value = (BM / WM) * (BN / WN) * ES > 64 * 64 * 8 ? 1 : 2;
+  static constexpr int lds = 2 * BK * (BM + PAD + BN + PAD) * ES;
+  static constexpr int value =
+      ((BM / WM) * (BN / WN) * ES > 64 * 64 * 8 || lds > 80 * 1024) ? 1 : 2;
 };
 
 template <class E, int BM, int BN, int BK, int WM, int WN, int PAD>
-__global__ __launch_bounds__(WM * WN * 64, (WavesPerSimd<BM, BN, WM, WN, (int)sizeof(E)>::value)) void
+__global__ __launch_bounds__(WM * WN * 64, (WavesPerSimd<BM, BN, BK, WM, WN, PAD, (int)sizeof(E)>::value)) void
 grouped_gemm_f64_kernel(
     const GemmProblem* __restrict__ probs, int nprob, const GemmSlice* __restrict__ slices,
     void* __restrict__ ws, XcdSplit xs) {
@@ -520,7 +523,7 @@ __device__ __forceinline__ void store_u(const SlabRegs<Tile<E, BM, BN, BK, WM, W
 }
 
 template <class E, int BM, int BN, int BK, int WM, int WN, int PAD, int TA>
-__global__ __launch_bounds__(WM * WN * 64, (WavesPerSimd<BM, BN, WM, WN, (int)sizeof(E)>::value)) void
+__global__ __launch_bounds__(WM * WN * 64, (WavesPerSimd<BM, BN, BK, WM, WN, PAD, (int)sizeof(E)>::value)) void
 grouped_gemm_f64_uniform(const GemmProblem* __restrict__ probs, int nprob,
                          const GemmSlice* __restrict__ slices, void* __restrict__ ws,
                          XcdSplit xs) {
