@@ -172,8 +172,8 @@ class Context:
 
     def set_svd_precondition(self, min_k=512):
         """QR preconditioning of the Jacobi SVD for sector blocks with
-        min(m, n) >= min_k (0 or False: off, the context's default; True:
-        512)."""
+        min(m, n) >= min_k (the context's default is 512; 0 or False: off;
+        True: 512)."""
         if min_k is True:
             min_k = 512
         self.check(lib.su2_ctx_set_svd_precondition(self.h, int(min_k)))
