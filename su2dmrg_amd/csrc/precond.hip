@@ -25,7 +25,6 @@ struct PDesc {
                // with its vectors (below X's diagonal)
   double* Q;   // K x K, column-major: Q2
   double* T;   // 256 per panel: the panel's 16 x 16 upper triangular T (column-major)
-  double* Z;   // 16 x K: T-scaled products V^T A of the panel being applied
   int K;
   double tiny; // Cholesky pivot floor
 };
@@ -306,13 +305,16 @@ __device__ __forceinline__ double hh_v(const PDesc& D, int c0, int bp, int m, in
   return D.G[(size_t)(c0 + r) * D.K + (c0 + i)];
 }
 
-// Z = op(T) (V^T A) for 16 columns of A (rows c0..K-1): A = X from column
-// c0 + bp (the QR's trailing columns, op(T) = T^T) or A = Q from column c0
-// (Q2's backward formation, op(T) = T).  1024 threads: four row groups of
-// 256 threads, each a 16 x 16 block of sums over its 64-row chunks (through
-// LDS, in order), then the four groups summed in order.
-__global__ __launch_bounds__(1024) void hh_dots_kernel(const PDesc* __restrict__ desc, int p,
-                                                       int qr) {
+// A <- A - V op(T) (V^T A) for 16 columns of A (rows c0..K-1): A = X from
+// column c0 + bp (the QR's trailing columns, op(T) = T^T) or A = Q from
+// column c0 (Q2's backward formation, op(T) = T).  One workgroup owns its 16
+// columns over every row, so the products and the update need no other
+// workgroup.  1024 threads: four row groups of 256 threads, each a 16 x 16
+// block of sums over its 64-row chunks (through LDS, in order), the four
+// groups summed in order, Z = op(T) sums in LDS; then each group updates
+// 64 x 16 tiles, each element's 16-term sum in order.
+__global__ __launch_bounds__(1024) void hh_apply_kernel(const PDesc* __restrict__ desc, int p,
+                                                        int qr) {
   const PDesc D = desc[blockIdx.y];
   const int K = D.K, c0 = PB * p;
   if (c0 >= K) return;
@@ -321,11 +323,12 @@ __global__ __launch_bounds__(1024) void hh_dots_kernel(const PDesc* __restrict__
   const int chunk = blockIdx.x * PB;
   if (chunk >= ncols) return;
   // A(row, col): X row-major (the QR's trailing columns) or Q column-major
-  const double* A = qr ? D.G : D.Q;
+  double* A = qr ? D.G : D.Q;
   const size_t rs = qr ? (size_t)K : 1, cs = qr ? 1 : (size_t)K;
   constexpr int NG = 4;
   __shared__ double sv[NG][64][PB + 1], sa[NG][64][PB + 1];
   __shared__ double sd[NG][PB][PB + 1];
+  __shared__ double sz[PB][PB + 1];
   const int t = threadIdx.x, g = t / 256, tg = t % 256;
   const int i = tg / PB, c = tg % PB;
   double acc = 0.0;
@@ -370,47 +373,30 @@ __global__ __launch_bounds__(1024) void hh_dots_kernel(const PDesc* __restrict__
     double z = 0.0;
 #pragma unroll
     for (int k = 0; k < PB; ++k) z = fma(qr ? T[k + PB * i] : T[i + PB * k], sd[0][k][c], z);
-    if (chunk + c < ncols) D.Z[(size_t)i + (size_t)PB * (chunk + c)] = z;
-  }
-}
-
-// A -= V Z on rows c0..K-1 of the same columns as hh_dots_kernel; 64 x 16
-// tiles, the 16-term sums in order.
-__global__ __launch_bounds__(256) void hh_update_kernel(const PDesc* __restrict__ desc, int p,
-                                                        int qr) {
-  const PDesc D = desc[blockIdx.z];
-  const int K = D.K, c0 = PB * p;
-  if (c0 >= K) return;
-  const int bp = min(PB, K - c0), m = K - c0;
-  const int colstart = c0 + (qr ? bp : 0), ncols = K - colstart;
-  const int chunk = blockIdx.x * PB, r0 = blockIdx.y * 64;
-  if (chunk >= ncols || r0 >= m) return;
-  double* A = qr ? D.G : D.Q;
-  const size_t rs = qr ? (size_t)K : 1, cs = qr ? 1 : (size_t)K;
-  __shared__ double sv[64][PB + 1];
-  __shared__ double sz[PB][PB + 1];
-  const int t = threadIdx.x;
-  for (int e = t; e < 64 * PB; e += 256) {
-    const int rr = e / PB, k = e % PB;
-    sv[rr][k] = hh_v(D, c0, bp, m, k, r0 + rr);
-  }
-  {
-    const int k = t % PB, cc = t / PB;
-    sz[k][cc] = (chunk + cc < ncols) ? D.Z[(size_t)k + (size_t)PB * (chunk + cc)] : 0.0;
+    sz[i][c] = (chunk + c < ncols) ? z : 0.0;
   }
   __syncthreads();
-  // X: a thread per column of a row (contiguous); Q: a thread per row of a
-  // column (contiguous)
+  // A -= V Z: X a thread per column of a row (contiguous), Q a thread per
+  // row of a column (contiguous)
+  for (int base = 0; base < m; base += 64 * NG) {
+    const int r0 = base + 64 * g;
+    for (int e = tg; e < 64 * PB; e += 256) {
+      const int rr = e / PB, k = e % PB;
+      sv[g][rr][k] = hh_v(D, c0, bp, m, k, r0 + rr);
+    }
+    __syncthreads();
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int rr = qr ? t / PB + 16 * s : t % 64;
-    const int cc = qr ? t % PB : t / 64 + 4 * s;
-    const int r = r0 + rr;
-    if (r >= m || chunk + cc >= ncols) continue;
-    double acc = 0.0;
+    for (int s = 0; s < 4; ++s) {
+      const int rr = qr ? tg / PB + 16 * s : tg % 64;
+      const int cc = qr ? tg % PB : tg / 64 + 4 * s;
+      const int r = r0 + rr;
+      if (r >= m || chunk + cc >= ncols) continue;
+      double a = 0.0;
 #pragma unroll
-    for (int k = 0; k < PB; ++k) acc = fma(sv[rr][k], sz[k][cc], acc);
-    A[(size_t)(c0 + r) * rs + (size_t)(colstart + chunk + cc) * cs] -= acc;
+      for (int k = 0; k < PB; ++k) a = fma(sv[g][rr][k], sz[k][cc], a);
+      A[(size_t)(c0 + r) * rs + (size_t)(colstart + chunk + cc) * cs] -= a;
+    }
+    __syncthreads();
   }
 }
 
@@ -437,7 +423,7 @@ void precondition_jacobi(Context& c, std::vector<PrecondBlock>& blocks) {
               "jacobi preconditioner: block %d x %d", b.L, b.K);
     kmax = std::max(kmax, b.K);
     const long long np = (b.K + PB - 1) / PB;
-    scratch += 2LL * b.K * b.K + np * PB * PB + (long long)PB * b.K;
+    scratch += 2LL * b.K * b.K + np * PB * PB;
   }
   DevBuf work(c, scratch);
   std::vector<PDesc> hd(nb);
@@ -449,8 +435,7 @@ void precondition_jacobi(Context& c, std::vector<PrecondBlock>& blocks) {
     d.G = work.p + off;
     d.Q = d.G + kk;
     d.T = d.Q + kk;
-    d.Z = d.T + np * PB * PB;
-    off += 2 * kk + np * PB * PB + (long long)PB * b.K;
+    off += 2 * kk + np * PB * PB;
     d.K = b.K;
     d.tiny = 1e-15 * b.normF * b.normF;
   }
@@ -481,9 +466,7 @@ void precondition_jacobi(Context& c, std::vector<PrecondBlock>& blocks) {
     hipLaunchKernelGGL(hh_panel_kernel, dim3(nb), dim3(PT), 0, st, dd, p);
     const int n = kmax - PB * (p + 1);
     if (n > 0) {
-      hipLaunchKernelGGL(hh_dots_kernel, dim3(cdiv(n, PB), nb), dim3(1024), 0, st, dd, p, 1);
-      hipLaunchKernelGGL(hh_update_kernel, dim3(cdiv(n, PB), cdiv(kmax - PB * p, 64), nb),
-                         dim3(256), 0, st, dd, p, 1);
+      hipLaunchKernelGGL(hh_apply_kernel, dim3(cdiv(n, PB), nb), dim3(1024), 0, st, dd, p, 1);
     }
   }
   // Q2 = H_0 ... H_{np-1}, formed backwards from the identity
@@ -491,9 +474,7 @@ void precondition_jacobi(Context& c, std::vector<PrecondBlock>& blocks) {
                      0, st, dd);
   for (int p = npmax - 1; p >= 0; --p) {
     const int n = kmax - PB * p;
-    hipLaunchKernelGGL(hh_dots_kernel, dim3(cdiv(n, PB), nb), dim3(1024), 0, st, dd, p, 0);
-    hipLaunchKernelGGL(hh_update_kernel, dim3(cdiv(n, PB), cdiv(n, 64), nb), dim3(256), 0, st, dd,
-                       p, 0);
+    hipLaunchKernelGGL(hh_apply_kernel, dim3(cdiv(n, PB), nb), dim3(1024), 0, st, dd, p, 0);
   }
   SU2_HIP(hipGetLastError());
   // W Q2 and V Q2
