@@ -78,7 +78,7 @@ int su2_ctx_memory(su2_ctx* ctx, long long* live, long long* peak, long long* ca
  * results agree to the SVD's own accuracy either way. */
 int su2_ctx_set_svd_tuning(su2_ctx* ctx, int block_cols, int inner_sweeps);
 /* QR preconditioning of the Jacobi SVD: for every sector block with
- * min_k <= min(m, n) <= 3072 (min_k 2..3072, default 512; 0 turns it off),
+ * min_k <= min(m, n) <= 2048 (min_k 2..2048, default 512; 0 turns it off),
  * the columns sorted by norm, the Gram matrix's Cholesky factor Rc and the
  * Householder QR Rc^T = Q2 R2 give an orthogonal Q2 with which the Jacobi
  * sweeps start (W <- W P Q2, V <- P Q2).  A block then needs fewer sweeps,

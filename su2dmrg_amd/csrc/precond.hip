@@ -15,7 +15,7 @@ namespace {
 
 constexpr int PB = 16;        // panel width
 constexpr int PT = 1024;      // threads of the panel kernels
-constexpr int PROWS = 3;      // rows (or columns) per thread of the panel kernels
+constexpr int PROWS = 2;      // rows (or columns) per thread of the panel kernels
 static_assert(PT * PROWS == kPrecondMaxK, "panel register tile");
 
 struct PDesc {
@@ -41,8 +41,8 @@ __device__ __forceinline__ void static_for(F&& f) {
 }
 
 // Upper Cholesky of G by one 16-row panel: rows c0..c0+bp-1, columns
-// c0..K-1, column j of the row panel on thread j % 1024 (the first two
-// thirds in registers, the last in LDS).  A pivot at or below tiny (or
+// c0..K-1, column j of the row panel on thread j % 1024, in registers.  A
+// pivot at or below tiny (or
 // negative from rounding) gives a zero row.  One barrier per step: row k is
 // published before its pivot is used, into alternating LDS buffers.
 __global__ __launch_bounds__(PT) void chol_panel_kernel(const PDesc* __restrict__ desc, int p) {
@@ -51,9 +51,8 @@ __global__ __launch_bounds__(PT) void chol_panel_kernel(const PDesc* __restrict_
   if (c0 >= K) return;
   const int bp = min(PB, K - c0), nc = K - c0;
   const int t = threadIdx.x;
-  __shared__ double lw[PT * (PB + 1)];   // third register column, padded rows
   __shared__ double s_raw[2][PB];
-  double w[2][PB];
+  double w[PROWS][PB];
   double* gcol[PROWS];
 #pragma unroll
   for (int q = 0; q < PROWS; ++q) {
@@ -66,10 +65,6 @@ __global__ __launch_bounds__(PT) void chol_panel_kernel(const PDesc* __restrict_
 #pragma unroll
     for (int i = 0; i < PB; ++i) w[q][i] = (j < nc && i < bp) ? gcol[q][i] : 0.0;
   }
-  double* l2 = lw + t * (PB + 1);
-  const int j2 = t + 2 * PT;
-#pragma unroll
-  for (int i = 0; i < PB; ++i) l2[i] = (j2 < nc && i < bp) ? gcol[2][i] : 0.0;
   static_for<0, PB>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
     __builtin_amdgcn_sched_barrier(0);
@@ -81,22 +76,22 @@ __global__ __launch_bounds__(PT) void chol_panel_kernel(const PDesc* __restrict_
       const double dk = raw[k];
       const double inv = dk > D.tiny ? 1.0 / sqrt(dk) : 0.0;
       // R[k][j] for my columns (zero left of the diagonal), then the rows
-      // below: w[i][j] -= R[k][i] R[k][j] for k < i <= j
+      // below: w[i][j] -= R[k][i] R[k][j] for k < i <= j (row by row, so
+      // that each R[k][i] is live for one row only)
+      double rkj[PROWS];
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
+      for (int q = 0; q < PROWS; ++q) {
         const int j = t + PT * q;
-        const double rkj = (j >= k) ? w[q][k] * inv : 0.0;
-        w[q][k] = rkj;
-#pragma unroll
-        for (int i = k + 1; i < PB; ++i)
-          if (i < bp && i <= j) w[q][i] = fma(-raw[i] * inv, rkj, w[q][i]);
+        rkj[q] = (j >= k) ? w[q][k] * inv : 0.0;
+        w[q][k] = rkj[q];
       }
-      {
-        const double rkj = l2[k] * inv;   // j2 >= 2048 > k
-        l2[k] = rkj;
 #pragma unroll
-        for (int i = k + 1; i < PB; ++i)
-          if (i < bp) l2[i] = fma(-raw[i] * inv, rkj, l2[i]);
+      for (int i = k + 1; i < PB; ++i) {
+        if (i >= bp) continue;
+        const double f = -raw[i] * inv;
+#pragma unroll
+        for (int q = 0; q < PROWS; ++q)
+          if (i <= t + PT * q) w[q][i] = fma(f, rkj[q], w[q][i]);
       }
     }
   });
@@ -107,11 +102,6 @@ __global__ __launch_bounds__(PT) void chol_panel_kernel(const PDesc* __restrict_
 #pragma unroll
     for (int i = 0; i < PB; ++i)
       if (i < bp) gcol[q][i] = (i > j) ? 0.0 : w[q][i];
-  }
-  if (j2 < nc) {
-#pragma unroll
-    for (int i = 0; i < PB; ++i)
-      if (i < bp) gcol[2][i] = l2[i];
   }
   // the panel's columns below its diagonal block: zero (Rc is upper
   // triangular, and read row-major they are X's entries above its diagonal)
@@ -155,7 +145,7 @@ __global__ __launch_bounds__(256) void chol_trail_kernel(const PDesc* __restrict
 }
 
 // Householder QR of one 16-column panel of X (rows c0..K-1; row c0 + r on
-// thread r % 1024, the first two thirds in registers, the last in LDS).
+// thread r % 1024, in registers).
 // Column j becomes its Householder vector below the diagonal (the unit
 // diagonal implicit); the panel's T factor of the compact WY form H_0 ...
 // H_{b-1} = I - V T V^T goes to D.T.  Each step takes one block reduction of
@@ -169,7 +159,6 @@ __global__ __launch_bounds__(PT) void hh_panel_kernel(const PDesc* __restrict__ 
   const int bp = min(PB, K - c0), m = K - c0;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
-  __shared__ double lx[PT * (PB + 1)];   // third register row, padded
   __shared__ double s_red[2][PT / 64][PB + 1];
   __shared__ double s_row[2][PB];
   __shared__ double s_d[2][PB];
@@ -187,10 +176,6 @@ __global__ __launch_bounds__(PT) void hh_panel_kernel(const PDesc* __restrict__ 
 #pragma unroll
     for (int c = 0; c < PB; ++c) x[s][c] = (r < m && c < bp) ? xrow[s][c] : 0.0;
   }
-  double* l2 = lx + t * (PB + 1);
-  const int r2 = t + 2 * PT;
-#pragma unroll
-  for (int c = 0; c < PB; ++c) l2[c] = (r2 < m && c < bp) ? xrow[2][c] : 0.0;
   if (t < PB * PB) s_T[t % PB][t / PB] = 0.0;
   static_for<0, PB>([&](auto jc) {
     constexpr int j = decltype(jc)::value;
@@ -211,7 +196,6 @@ __global__ __launch_bounds__(PT) void hh_panel_kernel(const PDesc* __restrict__ 
             const int r = t + PT * s;
             if (r > j && r < m) v[c] = fma(x[s][j], x[s][c], v[c]);
           }
-          if (r2 > j && r2 < m) v[c] = fma(l2[j], l2[c], v[c]);
         }
       }
       static_assert(PB == 16, "reduce-scatter of 16 sums over a 64-lane wave");
@@ -260,7 +244,11 @@ __global__ __launch_bounds__(PT) void hh_panel_kernel(const PDesc* __restrict__ 
         s_T[t][j] = -tau * acc;
       }
       if (t == j) s_T[j][j] = tau;
-      // v_j into column j, H_j applied to the later columns (rows >= j)
+      // v_j into column j, H_j applied to the later columns (rows >= j);
+      // w_c = v_j^T x_c read from LDS once for all rows
+      double wc[PB];
+#pragma unroll
+      for (int c = j + 1; c < PB; ++c) wc[c] = fma(scale, s_d[b][c], s_row[b][c]);
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
         const int r = t + PT * s;
@@ -269,15 +257,8 @@ __global__ __launch_bounds__(PT) void hh_panel_kernel(const PDesc* __restrict__ 
           if (r > j) x[s][j] = vr;
 #pragma unroll
           for (int c = j + 1; c < PB; ++c)
-            if (c < bp) x[s][c] = fma(-tau * vr, fma(scale, s_d[b][c], s_row[b][c]), x[s][c]);
+            if (c < bp) x[s][c] = fma(-tau * vr, wc[c], x[s][c]);
         }
-      }
-      if (r2 > j && r2 < m) {
-        const double vr = l2[j] * scale;
-        l2[j] = vr;
-#pragma unroll
-        for (int c = j + 1; c < PB; ++c)
-          if (c < bp) l2[c] = fma(-tau * vr, fma(scale, s_d[b][c], s_row[b][c]), l2[c]);
       }
     }
   });
@@ -289,11 +270,6 @@ __global__ __launch_bounds__(PT) void hh_panel_kernel(const PDesc* __restrict__ 
 #pragma unroll
     for (int c = 0; c < PB; ++c)
       if (c < bp) xrow[s][c] = x[s][c];
-  }
-  if (r2 < m) {
-#pragma unroll
-    for (int c = 0; c < PB; ++c)
-      if (c < bp) xrow[2][c] = l2[c];
   }
   if (t < PB * PB) D.T[(size_t)p * PB * PB + t] = s_T[t % PB][t / PB];
 }

@@ -35,8 +35,9 @@ struct PrecondBlock {
   double normF;     // Frobenius norm of W (the Cholesky pivot floor is 1e-15 normF^2)
 };
 
-// Largest K the panel kernels take (three rows per thread of 1024).
-constexpr int kPrecondMaxK = 3072;
+// Largest K the panel kernels take (two rows per thread of 1024, held in
+// registers; a third row would not fit them and spilled).
+constexpr int kPrecondMaxK = 2048;
 
 // Preconditions every block (each must have 1 <= K <= kPrecondMaxK and
 // L >= K): Wout = W Q2, Vout = V Q2.  Blocks never affect each other's bits.
