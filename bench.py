@@ -360,6 +360,10 @@ def cpu_sweep_leg(L, ctx, lx, ly, D):
             "gpu_sweep_s": round(t_gpu, 3), "ratio": round(t_cpu / t_gpu, 2),
             "cpu_energy_per_site": 0.5 * e_cpu / (lx * ly),
             "gpu_energy_per_site": 0.5 * e_gpu / (lx * ly),
+            "energy_note": "a timing comparison: on this symmetric cylinder the warm-up keeps "
+                           "sectors of rounding-noise weight, so the two trajectories part after "
+                           "it; energy parity is tested on well-conditioned lattices and against "
+                           "exact diagonalisation",
             "cpu": f"oracle (numpy/OpenBLAS, {model}, {ncpu} cores), one sweep after its own "
                    "warm-up"}
 
