@@ -26,6 +26,7 @@ struct PDesc {
   double* Q;   // K x K, column-major: Q2
   double* T;   // 256 per panel: the panel's 16 x 16 upper triangular T (column-major)
   int K;
+  int rk;      // rows of Rc up to its last nonzero pivot (X's later columns are zero)
   double tiny; // Cholesky pivot floor
 };
 
@@ -155,7 +156,8 @@ __global__ __launch_bounds__(256) void chol_trail_kernel(const PDesc* __restrict
 __global__ __launch_bounds__(PT) void hh_panel_kernel(const PDesc* __restrict__ desc, int p) {
   const PDesc D = desc[blockIdx.x];
   const int K = D.K, c0 = PB * p;
-  if (c0 >= K) return;
+  // a panel of zero columns (past the last nonzero pivot) is the identity
+  if (c0 >= D.rk) return;
   const int bp = min(PB, K - c0), m = K - c0;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
@@ -293,9 +295,10 @@ __global__ __launch_bounds__(1024) void hh_apply_kernel(const PDesc* __restrict_
                                                         int qr) {
   const PDesc D = desc[blockIdx.y];
   const int K = D.K, c0 = PB * p;
-  if (c0 >= K) return;
+  if (c0 >= D.rk) return;   // identity reflectors
   const int bp = min(PB, K - c0), m = K - c0;
-  const int colstart = c0 + (qr ? bp : 0), ncols = K - colstart;
+  // X's columns from rk on are zero and stay zero
+  const int colstart = c0 + (qr ? bp : 0), ncols = (qr ? D.rk : K) - colstart;
   const int chunk = blockIdx.x * PB;
   if (chunk >= ncols) return;
   // A(row, col): X row-major (the QR's trailing columns) or Q column-major
@@ -376,6 +379,24 @@ __global__ __launch_bounds__(1024) void hh_apply_kernel(const PDesc* __restrict_
   }
 }
 
+// rk of every block: one past the last nonzero diagonal entry of Rc (a
+// dropped pivot leaves its whole row exactly zero)
+__global__ __launch_bounds__(256) void rank_prefix_kernel(const PDesc* __restrict__ desc,
+                                                          int* __restrict__ rk) {
+  const PDesc D = desc[blockIdx.x];
+  __shared__ int s[256];
+  int last = -1;
+  for (int i = threadIdx.x; i < D.K; i += 256)
+    if (D.G[(size_t)i + (size_t)i * D.K] != 0.0) last = max(last, i);
+  s[threadIdx.x] = last;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] = max(s[threadIdx.x], s[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) rk[blockIdx.x] = s[0] + 1;
+}
+
 __global__ void eye_batched_kernel(const PDesc* __restrict__ desc) {
   const PDesc D = desc[blockIdx.y];
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -413,6 +434,7 @@ void precondition_jacobi(Context& c, std::vector<PrecondBlock>& blocks) {
     d.T = d.Q + kk;
     off += 2 * kk + np * PB * PB;
     d.K = b.K;
+    d.rk = b.K;
     d.tiny = 1e-15 * b.normF * b.normF;
   }
   DevBuf ddesc(c, (long long)((sizeof(PDesc) * nb + 7) / 8));
@@ -437,10 +459,26 @@ void precondition_jacobi(Context& c, std::vector<PrecondBlock>& blocks) {
       hipLaunchKernelGGL(chol_trail_kernel, dim3(nt, nt, nb), dim3(256), 0, st, dd, p);
     }
   }
+  // Rc's rank prefix: X = Rc^T has zero columns from rk on (a rank-deficient
+  // block, such as a two-site wavefunction sector, drops its last pivots),
+  // whose panels are identity reflectors and whose updates change nothing;
+  // one wait brings the counts back to bound the panel loops
+  {
+    DevBuf drk(c, (nb + 1) / 2);
+    hipLaunchKernelGGL(rank_prefix_kernel, dim3(nb), dim3(256), 0, st, dd, (int*)drk.p);
+    std::vector<int> rk(nb);
+    SU2_HIP(hipMemcpyAsync(rk.data(), drk.p, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+    SU2_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < nb; ++i) hd[i].rk = rk[i];
+    SU2_HIP(hipMemcpyAsync(ddesc.p, hd.data(), sizeof(PDesc) * nb, hipMemcpyHostToDevice, st));
+  }
+  int rkmax = 0;
+  for (int i = 0; i < nb; ++i) rkmax = std::max(rkmax, hd[i].rk);
+  const int npq = (rkmax + PB - 1) / PB;
   // Householder QR of Rc^T by column panels (trailing columns updated)
-  for (int p = 0; p < npmax; ++p) {
+  for (int p = 0; p < npq; ++p) {
     hipLaunchKernelGGL(hh_panel_kernel, dim3(nb), dim3(PT), 0, st, dd, p);
-    const int n = kmax - PB * (p + 1);
+    const int n = rkmax - PB * (p + 1);
     if (n > 0) {
       hipLaunchKernelGGL(hh_apply_kernel, dim3(cdiv(n, PB), nb), dim3(1024), 0, st, dd, p, 1);
     }
@@ -448,7 +486,7 @@ void precondition_jacobi(Context& c, std::vector<PrecondBlock>& blocks) {
   // Q2 = H_0 ... H_{np-1}, formed backwards from the identity
   hipLaunchKernelGGL(eye_batched_kernel, dim3(cdiv((long long)kmax * kmax, 256), nb), dim3(256),
                      0, st, dd);
-  for (int p = npmax - 1; p >= 0; --p) {
+  for (int p = npq - 1; p >= 0; --p) {
     const int n = kmax - PB * p;
     hipLaunchKernelGGL(hh_apply_kernel, dim3(cdiv(n, PB), nb), dim3(1024), 0, st, dd, p, 0);
   }
