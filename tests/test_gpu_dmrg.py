@@ -130,6 +130,47 @@ def test_svd_preconditioned_equals_plain(ctx, m, n, decades):
                                atol=1e-12 * ref[-1])
 
 
+@pytest.mark.parametrize("m,n,rank,gap", [(700, 720, 400, True), (530, 512, 300, False)])
+def test_svd_rank_deficient_blocks(ctx, m, n, rank, gap):
+    # A two-site wavefunction sector is rank-deficient: its null directions
+    # reach the rounding level after preconditioning, leave the sweeps
+    # (deflation), and Rc's zero rows end the Householder QR early.  Values
+    # against LAPACK, the reconstruction, orthonormality of U over the
+    # nonzero values, and the preconditioned run against the plain one.
+    # With a gap the nonzero values stop at 1e-10 (the null space is at
+    # rounding level, as in a wavefunction sector); without one they run
+    # down to 1e-14, below the Cholesky pivot floor.
+    from su2dmrg_amd import lib as L
+    _say(f"svd {m}x{n} of rank {rank}, gap {gap}")
+    rng = np.random.default_rng(m + 7 * n)
+    u, _ = np.linalg.qr(rng.standard_normal((m, rank)))
+    v, _ = np.linalg.qr(rng.standard_normal((n, rank)))
+    A = (u * np.logspace(0, -10 if gap else -14, rank)) @ v.T
+    ref = np.sort(np.linalg.svd(A, compute_uv=False))
+    out = {}
+    for pre in (True, False):
+        ctx.set_svd_precondition(64 if pre else 0)
+        t = L.wavefunction(ctx, [1], [m], [n], {1: A})
+        out[pre] = L.svd_truncate(ctx, t, 0, 1, 100000)
+    ctx.set_svd_precondition(512)   # the context's default
+    for pre in (True, False):
+        U, V, kept, _ = out[pre]
+        s = np.array(kept)
+        assert len(s) == min(m, n)
+        np.testing.assert_allclose(s, ref, rtol=0, atol=1e-12 * ref[-1])
+        cut = s[s <= 1e-11 * s[-1]].sum()
+        np.testing.assert_allclose(U[1] @ V[1].T, A, rtol=0, atol=1e-13 * ref[-1] + cut)
+        big = s > 1e-11 * s[-1]   # U's columns below are zero by convention
+        if gap:
+            assert big.sum() == rank
+        Ub = U[1][:, big]
+        sb = s[big]
+        bound = 1e-13 * s[-1] / np.minimum.outer(sb, sb) + 1e-13
+        assert np.all(np.abs(Ub.T @ Ub - np.eye(big.sum())) <= bound)
+    np.testing.assert_allclose(np.array(out[True][2]), np.array(out[False][2]), rtol=0,
+                               atol=1e-12 * ref[-1])
+
+
 def _run_both(ctx, lx, ly, jcoup, bdry, D, nsweep, totspin=0):
     from su2dmrg_amd import lib as L
     lat = od.Lattice(lx, ly, jcoup=jcoup, bdry=bdry)
