@@ -160,7 +160,15 @@ void HeffBatch::apply(const std::vector<Req>& reqs) {
         throw;
       }
       SU2_HIP(hipStreamEndCapture(s, &P.graph));
-      SU2_HIP(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0));
+      hipError_t e = hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0);
+      if (e == hipErrorOutOfMemory) {
+        // the device memory caches of every context on this device give
+        // back their free blocks, then one more try
+        (void)hipGetLastError();
+        DevicePool::release_device(c_.device());
+        e = hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0);
+      }
+      SU2_HIP(e);
       ++graphs_built_;
     }
     SU2_HIP(hipGraphLaunch(P.exec, c_.stream()));

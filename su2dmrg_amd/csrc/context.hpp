@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <mutex>
 #include <memory>
 #include <string>
 #include <unordered_map>
@@ -30,9 +31,15 @@ namespace su2 {
 // Objects that may outlive their context (tensors, GEMM and H_eff plans)
 // share ownership of the pool: once the context is gone (detach), the pool
 // hands back what it holds and frees later blocks with hipFree directly.
+//
+// The pools of a process form a registry: when hipMalloc (or a HIP graph's
+// instantiation) runs out of device memory, the free blocks of every pool on
+// that device go back to the driver (each after its own stream's wait), so
+// that one context's cache does not starve another's allocations.  A pool
+// is guarded by its own mutex, so that another thread can release it.
 class DevicePool {
  public:
-  DevicePool(int device, hipStream_t stream) : device_(device), stream_(stream) {}
+  DevicePool(int device, hipStream_t stream);
   ~DevicePool();
   DevicePool(const DevicePool&) = delete;
   DevicePool& operator=(const DevicePool&) = delete;
@@ -41,10 +48,25 @@ class DevicePool {
   void set_stream(hipStream_t s);
   void detach() noexcept;
   // bytes handed out now, the most handed out at once, bytes held by the cache
-  size_t live_bytes() const { return live_bytes_; }
-  size_t peak_bytes() const { return peak_bytes_; }
-  size_t cached_bytes() const { return cached_bytes_; }
-  void reset_peak() { peak_bytes_ = live_bytes_; }
+  size_t live_bytes() const {
+    std::lock_guard<std::mutex> g(mu_);
+    return live_bytes_;
+  }
+  size_t peak_bytes() const {
+    std::lock_guard<std::mutex> g(mu_);
+    return peak_bytes_;
+  }
+  size_t cached_bytes() const {
+    std::lock_guard<std::mutex> g(mu_);
+    return cached_bytes_;
+  }
+  void reset_peak() {
+    std::lock_guard<std::mutex> g(mu_);
+    peak_bytes_ = live_bytes_;
+  }
+  // free blocks of every pool on the device back to the driver (pools busy
+  // in another thread at that moment are skipped)
+  static void release_device(int device);
 
  private:
   void drain();          // wait for the stream: no queued work uses a cached block
@@ -53,6 +75,7 @@ class DevicePool {
   int device_;
   hipStream_t stream_;
   bool detached_ = false;
+  mutable std::mutex mu_;
   std::multimap<size_t, void*> free_;           // size -> block, reusable in stream order
   std::unordered_map<void*, size_t> live_;      // handed out
   size_t live_bytes_ = 0, peak_bytes_ = 0, high_water_ = 0, cached_bytes_ = 0;

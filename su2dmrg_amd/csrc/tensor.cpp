@@ -45,17 +45,52 @@ size_t pool_round(size_t b) {
                                                                          : (size_t(2) << 20);
   return (b + g - 1) / g * g;
 }
+
+// every pool of the process (for releases when device memory runs out)
+std::mutex& pool_registry_mutex() {
+  static std::mutex m;
+  return m;
+}
+std::vector<DevicePool*>& pool_registry() {
+  static std::vector<DevicePool*> v;
+  return v;
+}
 }  // namespace
 
+DevicePool::DevicePool(int device, hipStream_t stream) : device_(device), stream_(stream) {
+  std::lock_guard<std::mutex> g(pool_registry_mutex());
+  pool_registry().push_back(this);
+}
+
 DevicePool::~DevicePool() {
+  {
+    std::lock_guard<std::mutex> g(pool_registry_mutex());
+    auto& r = pool_registry();
+    r.erase(std::remove(r.begin(), r.end(), this), r.end());
+  }
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto& x : free_) (void)hipFree(x.second);
   for (auto& x : live_) (void)hipFree(x.first);
 }
 
+void DevicePool::release_device(int device) {
+  std::lock_guard<std::mutex> g(pool_registry_mutex());
+  for (DevicePool* p : pool_registry()) {
+    if (p->device_ != device) continue;
+    std::unique_lock<std::mutex> l(p->mu_, std::try_to_lock);
+    if (!l.owns_lock() || p->free_.empty()) continue;
+    // no throw here: a failed wait leaves that pool's cache as it is
+    if (p->stream_ && hipStreamSynchronize(p->stream_) != hipSuccess) continue;
+    for (auto& x : p->free_) (void)hipFree(x.second);
+    p->free_.clear();
+    p->cached_bytes_ = 0;
+  }
+}
+
 void* DevicePool::alloc(size_t bytes) {
   if (bytes == 0) return nullptr;
+  std::unique_lock<std::mutex> lock(mu_);
   SU2_CHECK(!detached_, kErrArgument, "device memory requested after its context was destroyed");
   const size_t sz = pool_round(bytes);
   auto take = [&]() -> void* {
@@ -75,7 +110,15 @@ void* DevicePool::alloc(size_t bytes) {
   if (hipMalloc(&p, sz) != hipSuccess) {
     (void)hipGetLastError();
     release_free();
-    SU2_HIP(hipMalloc(&p, sz));
+    if (hipMalloc(&p, sz) != hipSuccess) {
+      // the other pools' caches on this device (without holding this one's
+      // lock, so that two threads out of memory cannot wait on each other)
+      (void)hipGetLastError();
+      lock.unlock();
+      release_device(device_);
+      lock.lock();
+      SU2_HIP(hipMalloc(&p, sz));
+    }
   }
   live_[p] = sz;
   live_bytes_ += sz;
@@ -86,6 +129,7 @@ void* DevicePool::alloc(size_t bytes) {
 
 void DevicePool::free(void* p) {
   if (!p) return;
+  std::lock_guard<std::mutex> lock(mu_);
   auto it = live_.find(p);
   if (detached_ || it == live_.end()) {   // context gone (or not a pool block)
     if (it != live_.end()) {
@@ -119,6 +163,7 @@ void DevicePool::trim() {
 }
 
 void DevicePool::set_stream(hipStream_t s) {
+  std::lock_guard<std::mutex> lock(mu_);
   drain();   // cached blocks may still be in use by work on the old stream
   stream_ = s;
 }
@@ -126,6 +171,7 @@ void DevicePool::set_stream(hipStream_t s) {
 void DevicePool::detach() noexcept {
   // runs from ~Context: a sticky device error must not throw here, so the
   // wait's status is dropped (su2_ctx_destroy waits and reports it first)
+  std::lock_guard<std::mutex> lock(mu_);
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto& x : free_) (void)hipFree(x.second);
