@@ -13,6 +13,8 @@ namespace su2 {
 
 namespace {
 
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
 constexpr int PB = 16;        // panel width
 constexpr int PT = 1024;      // threads of the panel kernels
 constexpr int PROWS = 2;      // rows (or columns) per thread of the panel kernels
@@ -287,10 +289,11 @@ __device__ __forceinline__ double hh_v(const PDesc& D, int c0, int bp, int m, in
 // column c0 + bp (the QR's trailing columns, op(T) = T^T) or A = Q from
 // column c0 (Q2's backward formation, op(T) = T).  One workgroup owns its 16
 // columns over every row, so the products and the update need no other
-// workgroup.  1024 threads: four row groups of 256 threads, each a 16 x 16
-// block of sums over its 64-row chunks (through LDS, in order), the four
-// groups summed in order, Z = op(T) sums in LDS; then each group updates
-// 64 x 16 tiles, each element's 16-term sum in order.
+// workgroup.  1024 threads: 256-row chunks of V and A staged through LDS,
+// each of the 16 waves accumulating V^T A over its 16 rows of every chunk
+// on the matrix cores (four 16x16x4 FP64 MFMAs), the 16 partial products
+// summed in wave order, Z = op(T) sums in LDS; then each group of 256
+// threads updates 64 x 16 tiles, each element's 16-term sum in order.
 __global__ __launch_bounds__(1024) void hh_apply_kernel(const PDesc* __restrict__ desc, int p,
                                                         int qr) {
   const PDesc D = desc[blockIdx.y];
@@ -306,11 +309,13 @@ __global__ __launch_bounds__(1024) void hh_apply_kernel(const PDesc* __restrict_
   const size_t rs = qr ? (size_t)K : 1, cs = qr ? 1 : (size_t)K;
   constexpr int NG = 4;
   __shared__ double sv[NG][64][PB + 1], sa[NG][64][PB + 1];
-  __shared__ double sd[NG][PB][PB + 1];
+  __shared__ double sp[16][PB][PB + 1];   // the waves' partial V^T A
   __shared__ double sz[PB][PB + 1];
+  static_assert(PB == 16 && NG * 4 == 16, "16 waves of 16 rows per 256-row chunk");
   const int t = threadIdx.x, g = t / 256, tg = t % 256;
   const int i = tg / PB, c = tg % PB;
-  double acc = 0.0;
+  const int lane = t & 63, wave = t >> 6, gw = wave >> 2, rw = (wave & 3) * 16;
+  f64x4 pacc = {0.0, 0.0, 0.0, 0.0};
   // every thread runs the same iterations (the barriers inside need that);
   // a group past the last row loads zeros
   for (int base = 0; base < m; base += 64 * NG) {
@@ -334,24 +339,32 @@ __global__ __launch_bounds__(1024) void hh_apply_kernel(const PDesc* __restrict_
       }
     }
     __syncthreads();
-#pragma unroll 8
-    for (int rr = 0; rr < 64; ++rr) acc = fma(sv[g][rr][i], sa[g][rr][c], acc);
+    // (V^T A)[a][b] += sum over 4 rows: lane l feeds V[row l/16][l%16] as
+    // the A operand and A[row l/16][l%16] as the B operand
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int rr = rw + 4 * j + (lane >> 4);
+      pacc = __builtin_amdgcn_mfma_f64_16x16x4f64(sv[gw][rr][lane & 15], sa[gw][rr][lane & 15],
+                                                  pacc, 0, 0, 0);
+    }
     __syncthreads();
   }
-  sd[g][i][c] = acc;
+  // result register r of lane l is row l/16 + 4 r, column l % 16
+#pragma unroll
+  for (int r = 0; r < 4; ++r) sp[wave][(lane >> 4) + 4 * r][lane & 15] = pacc[r];
   __syncthreads();
   if (g == 0) {
     double v = 0.0;
 #pragma unroll
-    for (int q = 0; q < NG; ++q) v += sd[q][i][c];
-    sd[0][i][c] = v;
+    for (int w = 0; w < 16; ++w) v += sp[w][i][c];
+    sp[0][i][c] = v;   // row i of wave 0's block is read by this thread only
   }
   __syncthreads();
   if (g == 0) {
     const double* T = D.T + (size_t)p * PB * PB;   // T[a][b] at a + 16 b
     double z = 0.0;
 #pragma unroll
-    for (int k = 0; k < PB; ++k) z = fma(qr ? T[k + PB * i] : T[i + PB * k], sd[0][k][c], z);
+    for (int k = 0; k < PB; ++k) z = fma(qr ? T[k + PB * i] : T[i + PB * k], sp[0][k][c], z);
     sz[i][c] = (chunk + c < ncols) ? z : 0.0;
   }
   __syncthreads();
