@@ -292,8 +292,8 @@ __device__ __forceinline__ double hh_v(const PDesc& D, int c0, int bp, int m, in
 // workgroup.  1024 threads: 256-row chunks of V and A staged through LDS,
 // each of the 16 waves accumulating V^T A over its 16 rows of every chunk
 // on the matrix cores (four 16x16x4 FP64 MFMAs), the 16 partial products
-// summed in wave order, Z = op(T) sums in LDS; then each group of 256
-// threads updates 64 x 16 tiles, each element's 16-term sum in order.
+// summed in wave order, Z = op(T) sums in LDS; then A -= V Z by the same
+// waves, a 16 x 16 tile each on the matrix cores.
 __global__ __launch_bounds__(1024) void hh_apply_kernel(const PDesc* __restrict__ desc, int p,
                                                         int qr) {
   const PDesc D = desc[blockIdx.y];
@@ -368,8 +368,9 @@ __global__ __launch_bounds__(1024) void hh_apply_kernel(const PDesc* __restrict_
     sz[i][c] = (chunk + c < ncols) ? z : 0.0;
   }
   __syncthreads();
-  // A -= V Z: X a thread per column of a row (contiguous), Q a thread per
-  // row of a column (contiguous)
+  // A -= V Z: each wave a 16 x 16 tile of the 256-row chunk on the matrix
+  // cores (lane l feeds V[row l%16][k] and Z[k][l%16], k = 4 j + l/16), the
+  // result subtracted from A (lane l: rows l/16 + 4 r, column l % 16)
   for (int base = 0; base < m; base += 64 * NG) {
     const int r0 = base + 64 * g;
     for (int e = tg; e < 64 * PB; e += 256) {
@@ -377,16 +378,19 @@ __global__ __launch_bounds__(1024) void hh_apply_kernel(const PDesc* __restrict_
       sv[g][rr][k] = hh_v(D, c0, bp, m, k, r0 + rr);
     }
     __syncthreads();
+    f64x4 y = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int rr = qr ? tg / PB + 16 * s : tg % 64;
-      const int cc = qr ? tg % PB : tg / 64 + 4 * s;
-      const int r = r0 + rr;
-      if (r >= m || chunk + cc >= ncols) continue;
-      double a = 0.0;
+    for (int j = 0; j < 4; ++j) {
+      const int k = 4 * j + (lane >> 4);
+      y = __builtin_amdgcn_mfma_f64_16x16x4f64(sv[gw][rw + (lane & 15)][k], sz[k][lane & 15], y,
+                                               0, 0, 0);
+    }
+    const int cc = lane & 15;
 #pragma unroll
-      for (int k = 0; k < PB; ++k) a = fma(sv[g][rr][k], sz[k][cc], a);
-      A[(size_t)(c0 + r) * rs + (size_t)(colstart + chunk + cc) * cs] -= a;
+    for (int r = 0; r < 4; ++r) {
+      const int row = base + 64 * gw + rw + (lane >> 4) + 4 * r;
+      if (row < m && chunk + cc < ncols)
+        A[(size_t)(c0 + row) * rs + (size_t)(colstart + chunk + cc) * cs] -= y[r];
     }
     __syncthreads();
   }
