@@ -78,14 +78,14 @@ int su2_ctx_memory(su2_ctx* ctx, long long* live, long long* peak, long long* ca
  * results agree to the SVD's own accuracy either way. */
 int su2_ctx_set_svd_tuning(su2_ctx* ctx, int block_cols, int inner_sweeps);
 /* QR preconditioning of the Jacobi SVD: for every sector block with
- * min_k <= min(m, n) <= 2048 (min_k 2..2048, default 512; 0 turns it off),
+ * min_k <= min(m, n) <= 2048 (min_k 2..2048, default 256; 0 turns it off),
  * the columns sorted by norm, the Gram matrix's Cholesky factor Rc and the
  * Householder QR Rc^T = Q2 R2 give an orthogonal Q2 with which the Jacobi
  * sweeps start (W <- W P Q2, V <- P Q2).  A block then needs fewer sweeps,
  * and a rank-deficient one (two-site wavefunction sectors typically are)
  * has its null directions in columns at the rounding level from the start,
- * which the sweeps leave out: a 16x8 D=2000 sweep takes 30.8 s instead of
- * 37.3 s on one box.
+ * which the sweeps leave out: a 16x8 D=2000 sweep takes about 26 s instead
+ * of 37 s.
  * The reference computes its sector SVDs through the Gram matrix and dsyev
  * (tensor.cpp:1472-1529); this setting changes results by rounding only. */
 int su2_ctx_set_svd_precondition(su2_ctx* ctx, int min_k);

@@ -112,7 +112,7 @@ class Context {
   int max_dcut = 8000;  // reference default (main_omp.cpp:15)
   int svd_block_cols = 32;   // one-sided Jacobi column block width (16 or 32)
   int svd_inner_sweeps = 1;  // its inner sweeps per visit of a pair (1..3)
-  int svd_precondition_min_k = 512;  // QR-precondition Jacobi SVD blocks from this size (0: off)
+  int svd_precondition_min_k = 256;  // QR-precondition Jacobi SVD blocks from this size (0: off)
 
  private:
   int device_;

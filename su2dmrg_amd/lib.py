@@ -170,12 +170,12 @@ class Context:
     def sync(self):
         self.check(lib.su2_ctx_synchronize(self.h))
 
-    def set_svd_precondition(self, min_k=512):
+    def set_svd_precondition(self, min_k=256):
         """QR preconditioning of the Jacobi SVD for sector blocks with
-        min(m, n) >= min_k (the context's default is 512; 0 or False: off;
-        True: 512)."""
+        min(m, n) >= min_k (the context's default is 256; 0 or False: off;
+        True: 256)."""
         if min_k is True:
-            min_k = 512
+            min_k = 256
         self.check(lib.su2_ctx_set_svd_precondition(self.h, int(min_k)))
 
     def set_svd_tuning(self, block_cols=32, inner_sweeps=1):

@@ -119,7 +119,7 @@ def test_svd_preconditioned_equals_plain(ctx, m, n, decades):
         ctx.set_svd_precondition(48 if pre else 0)
         t = L.wavefunction(ctx, [1], [m], [n], {1: A})
         out[pre] = L.svd_truncate(ctx, t, 0, 1, 100000)
-    ctx.set_svd_precondition(512)   # the context's default
+    ctx.set_svd_precondition(256)   # the context's default
     for pre in (True, False):
         U, V, kept, _ = out[pre]
         s = np.array(kept)
@@ -152,7 +152,7 @@ def test_svd_rank_deficient_blocks(ctx, m, n, rank, gap):
         ctx.set_svd_precondition(64 if pre else 0)
         t = L.wavefunction(ctx, [1], [m], [n], {1: A})
         out[pre] = L.svd_truncate(ctx, t, 0, 1, 100000)
-    ctx.set_svd_precondition(512)   # the context's default
+    ctx.set_svd_precondition(256)   # the context's default
     for pre in (True, False):
         U, V, kept, _ = out[pre]
         s = np.array(kept)
